@@ -1,0 +1,58 @@
+"""Build recipes for the in-tree shared libraries (hipcc for gfx950, g++ for host code)."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libfftso3_amd.so")
+FDCL_LIB = os.path.join(HERE, "libfftso3_fdcl.so")
+
+HIP_SOURCES = [os.path.join(HERE, "csrc", "api.hip")]
+HIP_DEPS = [os.path.join(HERE, "csrc", f) for f in ("common.hpp", "fft.hpp", "kernels.hpp", "mfma.hpp")]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # keep the compiler from forming packed-FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build_hip(verbose=False):
+    """Compile the HIP kernels and the C ABI into fftso3_amd/libfftso3_amd.so."""
+    out = _run([HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", LIB] + HIP_SOURCES)
+    if verbose and out.strip():
+        print(out)
+    return LIB
+
+
+def build_fdcl(verbose=False):
+    """Compile the C++ class layer (fdcl:: drop-in) into fftso3_amd/libfftso3_fdcl.so."""
+    src_dir = os.path.join(HERE, "cpp", "src")
+    srcs = sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir) if f.endswith(".cpp"))
+    cmd = ["g++", "-O2", "-std=c++14", "-fPIC", "-shared", "-fopenmp",
+           "-I", os.path.join(HERE, "cpp", "include"), "-I", os.path.join(ROOT, "include"),
+           "-o", FDCL_LIB] + srcs + ["-L", HERE, "-lfftso3_amd", "-Wl,-rpath,$ORIGIN"]
+    out = _run(cmd)
+    if verbose and out.strip():
+        print(out)
+    return FDCL_LIB
+
+
+def build_all(verbose=False):
+    build_hip(verbose)
+    if os.path.isdir(os.path.join(HERE, "cpp", "src")) and any(
+            f.endswith(".cpp") for f in os.listdir(os.path.join(HERE, "cpp", "src"))):
+        build_fdcl(verbose)

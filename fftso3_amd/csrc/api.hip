@@ -1,0 +1,771 @@
+// api.hip -- plans, chunked execution and the C ABI of libfftso3_amd.so (gfx950).
+#include "kernels.hpp"
+
+#include "../../include/fftso3_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(FFTSO3_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+constexpr int kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
+constexpr int kFftThreads = 1024;
+constexpr int kNumStages = 6;
+
+// device buffer owned by a plan
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes)
+    {
+        n = bytes;
+        if (bytes == 0) return FFTSO3_OK;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(FFTSO3_ENOMEM, "hipMalloc(%zu) failed", bytes);
+        }
+        return FFTSO3_OK;
+    }
+    template <typename V> int upload(const std::vector<V> &v)
+    {
+        int st = alloc(v.size() * sizeof(V));
+        if (st) return st;
+        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+        return FFTSO3_OK;
+    }
+};
+
+// One bandlimit within a plan
+struct Class {
+    int B = 0, N = 0;
+    std::vector<int> fns;           // global function indices with this bandlimit
+    DevBuf d_fn, d_ident, d_ws_off;
+    DevBuf d_tw32, d_tw64, d_rev;   // twiddles and DIF digit reversal
+    FftDesc fd{};
+    // recursion tables
+    DevBuf d_cb, d_lnc, d_lns, d_w, d_lnbin, d_sq, d_isq;
+    DevBuf d_items_fwd, d_items_inv;
+    int nitems_fwd = 0, nitems_inv = 0, lt_fwd = 0, kt_inv = 0;
+    bool fused = false;             // whole slice fits in LDS
+    int row_tile = 0, col_tile = 0; // two-pass fallback tiles
+};
+
+}  // namespace
+
+struct fftso3_plan_s {
+    int domain = 0, kind = 0, prec = 0, n_fn = 0;
+    int C = 8;                         // chunk capacity (functions per pipeline pass)
+    size_t esz = 8;                    // bytes per complex element of storage (8 or 16)
+    std::vector<int> Bs;
+    std::vector<long> grid_off, coef_off;
+    long grid_total = 0, coef_total = 0;
+    DevBuf d_grid_off, d_coef_off;
+    std::vector<Class *> classes;
+    DevBuf d_ws;        // group-major spectrum workspace
+    DevBuf d_scratch;   // two-pass FFT scratch
+    DevBuf d_fc;        // complex coefficients of the real path
+    size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    std::vector<std::pair<int, int>> ev_used;   // (stage, pool index)
+    double acc_ms[kNumStages] = {};
+    long acc_n[kNumStages] = {};
+    ~fftso3_plan_s()
+    {
+        for (auto *c : classes) delete c;
+        for (auto &e : ev_pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------------------------
+FftDesc make_fft_desc(int N, std::vector<int> &rev)
+{
+    FftDesc fd{};
+    fd.N = N;
+    int n = N, e = 0;
+    while (n % 2 == 0) { n /= 2; e++; }
+    std::vector<int> rad;
+    int n8 = e / 3, r = e % 3;
+    if (r == 1 && n8 >= 1) { n8 -= 1; for (int i = 0; i < n8; i++) rad.push_back(8); rad.push_back(4); rad.push_back(4); }
+    else {
+        for (int i = 0; i < n8; i++) rad.push_back(8);
+        if (r == 2) rad.push_back(4);
+        if (r == 1) rad.push_back(2);
+    }
+    for (int p : {3, 5, 7})
+        while (n % p == 0) { n /= p; rad.push_back(p); }
+    rev.assign(N, 0);
+    if (n != 1 || (int)rad.size() > FFT_MAX_STAGES) {  // prime factor > 7: direct DFT
+        fd.nst = 0;
+        for (int f = 0; f < N; f++) rev[f] = f;
+        return fd;
+    }
+    fd.nst = (int)rad.size();
+    int L = N;
+    for (int t = 0; t < fd.nst; t++) {
+        fd.R[t] = rad[t];
+        fd.M[t] = L / rad[t];
+        L = fd.M[t];
+    }
+    for (int f = 0; f < N; f++) {
+        int rem = f, pos = 0;
+        for (int t = 0; t < fd.nst; t++) {
+            pos += (rem % fd.R[t]) * fd.M[t];
+            rem /= fd.R[t];
+        }
+        rev[f] = pos;
+    }
+    return fd;
+}
+
+int build_class_tables(fftso3_plan_s *pl, Class *c)
+{
+    const int B = c->B, N = c->N;
+    const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    const long double PI = 3.141592653589793238462643383279502884L;
+    int st;
+    // twiddles W_N^t
+    std::vector<float2> tw32(N);
+    std::vector<double2> tw64(N);
+    for (int t = 0; t < N; t++) {
+        long double a = -2.0L * PI * (long double)t / (long double)N;
+        tw64[t] = make_double2((double)cosl(a), (double)sinl(a));
+        tw32[t] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+    std::vector<int> rev;
+    c->fd = make_fft_desc(N, rev);
+    if ((st = c->d_tw32.upload(tw32)) || (st = c->d_tw64.upload(tw64)) || (st = c->d_rev.upload(rev))) return st;
+
+    // grid and recursion tables (FP64, evaluated in long double)
+    std::vector<double> cb(N), lnc(N), lns(N), w(N);
+    for (int k = 0; k < N; k++) {
+        long double beta = (long double)(2 * k + 1) * PI / (4.0L * B);  // src/fdcl_FFTSO3.cpp:578-581
+        cb[k] = (double)cosl(beta);
+        lnc[k] = (double)logl(cosl(beta / 2));
+        lns[k] = s2 ? (double)logl(sinl(beta)) : (double)logl(sinl(beta / 2));
+        // weights: src/fdcl_FFTSO3.cpp:236-256 (SO(3)), src/fdcl_FFTS2.cpp:102-123 (S^2)
+        long double sum = 0;
+        for (int i = 0; i < B; i++)
+            sum += sinl((long double)((2 * k + 1) * (2 * i + 1)) * PI / (4.0L * B)) / (long double)(2 * i + 1);
+        long double pref = s2 ? 2.0L * PI / ((long double)B * B) : 1.0L / (4.0L * B * B * B);
+        w[k] = (double)(pref * sinl(beta) * sum);
+    }
+    std::vector<double> sq((size_t)B * B, 0.0), isq((size_t)B * B, 0.0);
+    std::vector<double> lnbin;
+    if (!s2) {
+        for (int l = 0; l < B; l++)
+            for (int m = 0; m <= l; m++) {
+                long double v = sqrtl((long double)l * l - (long double)m * m);
+                sq[(size_t)l * B + m] = (double)v;
+                isq[(size_t)l * B + m] = m < l ? (double)(1.0L / v) : 0.0;
+            }
+        lnbin.resize((size_t)B * B);
+        for (int m0 = 0; m0 < B; m0++)
+            for (int n0 = -m0; n0 <= m0; n0++)
+                lnbin[(size_t)m0 * m0 + n0 + m0] =
+                    (double)(0.5L * (lgammal(2.0L * m0 + 1) - lgammal((long double)(m0 + n0) + 1) -
+                                     lgammal((long double)(m0 - n0) + 1)));
+    } else {
+        for (int l = 0; l < B; l++)
+            for (int m = 0; m < l; m++) {
+                long double f = sqrtl((4.0L * l * l - 1.0L) / ((long double)l * l - (long double)m * m));
+                sq[(size_t)l * B + m] = (double)f;
+                isq[(size_t)l * B + m] = (double)(1.0L / f);
+            }
+        lnbin.resize(B);
+        long double lp = 0;  // ln prod_{i<=m} (2i-1)/(2i)
+        for (int m = 0; m < B; m++) {
+            if (m > 0) lp += logl((2.0L * m - 1.0L) / (2.0L * m));
+            lnbin[m] = (double)(0.5L * (logl((2.0L * m + 1.0L) / (4.0L * PI)) + lp));
+        }
+    }
+    if ((st = c->d_cb.upload(cb)) || (st = c->d_lnc.upload(lnc)) || (st = c->d_lns.upload(lns)) ||
+        (st = c->d_w.upload(w)) || (st = c->d_lnbin.upload(lnbin)) || (st = c->d_sq.upload(sq)) ||
+        (st = c->d_isq.upload(isq)))
+        return st;
+    return FFTSO3_OK;
+}
+
+int build_class_work(fftso3_plan_s *pl, Class *c)
+{
+    const int B = c->B, N = c->N, C = pl->C;
+    const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    const size_t tsz = pl->esz / 2;  // real element size
+    int st;
+    // contraction work lists
+    const int np = s2 ? 2 : 4;
+    const int ncol = 2 * np * C;
+    const int ngroups = 32 * C / ncol;
+    c->lt_fwd = 4 * ngroups;
+    const int ngrp = s2 ? B : B * B;
+    std::vector<int2> itf, iti;
+    for (int g = 0; g < ngrp; g++) {
+        int m0 = s2 ? g : (int)std::sqrt((double)g + 0.5);
+        if (!s2) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
+        for (int l = m0; l < B; l += c->lt_fwd) itf.push_back(make_int2(g, l));
+    }
+    // inverse: k tile sized so that D[B][KT] + Fv[B][ncol] stays within 64 KB of LDS
+    int kt = N;
+    while (kt > 4 && (size_t)B * (kt + ncol) * tsz > 64 * 1024) kt /= 2;
+    c->kt_inv = kt;
+    for (int g = 0; g < ngrp; g++)
+        for (int k = 0; k < N; k += kt) iti.push_back(make_int2(g, k));
+    c->nitems_fwd = (int)itf.size();
+    c->nitems_inv = (int)iti.size();
+    if ((st = c->d_items_fwd.upload(itf)) || (st = c->d_items_inv.upload(iti))) return st;
+    std::vector<int> ident(C);
+    for (int i = 0; i < C; i++) ident[i] = i;
+    std::vector<long> wsoff(C);
+    const long nc = s2 ? (long)B * B : so3_ncoef(B);
+    for (int i = 0; i < C; i++) wsoff[i] = (long)i * nc;
+    if ((st = c->d_ident.upload(ident)) || (st = c->d_ws_off.upload(wsoff))) return st;
+    // FFT mode
+    size_t slice = (size_t)N * N * pl->esz;
+    c->fused = !s2 && slice <= (size_t)kMaxLds && (size_t)N * N <= (size_t)FFT_EPT * kFftThreads;
+    const size_t cap = std::min((size_t)kMaxLds / pl->esz, (size_t)FFT_EPT * kFftThreads);
+    int rt = 1;
+    while (rt * 2 <= N && N % (rt * 2) == 0 && (size_t)(rt * 2) * N <= cap) rt *= 2;
+    c->row_tile = rt;
+    c->col_tile = rt;
+    return FFTSO3_OK;
+}
+
+int finalize_plan(fftso3_plan_s *pl)
+{
+    int st;
+    for (auto *c : pl->classes) delete c;
+    pl->classes.clear();
+    std::map<int, Class *> byB;
+    for (int i = 0; i < pl->n_fn; i++) {
+        int B = pl->Bs[i];
+        if (!byB.count(B)) {
+            Class *c = new (std::nothrow) Class();
+            if (!c) return fail(FFTSO3_ENOMEM, "host allocation failed");
+            c->B = B;
+            c->N = 2 * B;
+            byB[B] = c;
+            pl->classes.push_back(c);
+        }
+        byB[B]->fns.push_back(i);
+    }
+    size_t ws = 0, scr = 0, fc = 0;
+    const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    for (auto *c : pl->classes) {
+        if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
+        if ((st = c->d_fn.upload(c->fns))) return st;
+        const size_t B = c->B, N = c->N, C = pl->C;
+        size_t w = s2 ? C * B * N * 2 * pl->esz : C * B * B * N * 4 * pl->esz;
+        ws = std::max(ws, w);
+        if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
+        if (pl->kind == FFTSO3_REAL)
+            fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
+    }
+    if (ws != pl->ws_bytes) { pl->d_ws.~DevBuf(); new (&pl->d_ws) DevBuf(); if ((st = pl->d_ws.alloc(ws))) return st; }
+    if (scr != pl->scratch_bytes) { pl->d_scratch.~DevBuf(); new (&pl->d_scratch) DevBuf(); if ((st = pl->d_scratch.alloc(scr))) return st; }
+    if (fc != pl->fc_bytes) { pl->d_fc.~DevBuf(); new (&pl->d_fc) DevBuf(); if ((st = pl->d_fc.alloc(fc))) return st; }
+    pl->ws_bytes = ws;
+    pl->scratch_bytes = scr;
+    pl->fc_bytes = fc;
+    return FFTSO3_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// launching
+// ---------------------------------------------------------------------------------------------
+struct StageTimer {
+    fftso3_plan_s *pl;
+    hipStream_t s;
+    int stage;
+    int idx = -1;
+    StageTimer(fftso3_plan_s *p, hipStream_t st, int sg) : pl(p), s(st), stage(sg)
+    {
+        if (!pl->timing) return;
+        idx = (int)pl->ev_used.size();
+        if (idx >= (int)pl->ev_pool.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { idx = -1; return; }
+            pl->ev_pool.push_back({a, b});
+        }
+        pl->ev_used.push_back({stage, idx});
+        (void)hipEventRecord(pl->ev_pool[idx].first, s);
+    }
+    ~StageTimer()
+    {
+        if (idx >= 0) (void)hipEventRecord(pl->ev_pool[idx].second, s);
+    }
+};
+
+template <typename T>
+ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
+{
+    ChunkArgs ca;
+    ca.B = c->B;
+    ca.N = c->N;
+    ca.C = pl->C;
+    ca.nfc = nfc;
+    ca.fn = reinterpret_cast<const int *>(c->d_fn.p) + i0;
+    ca.grid_off = reinterpret_cast<const long *>(pl->d_grid_off.p);
+    ca.coef_off = reinterpret_cast<const long *>(pl->d_coef_off.p);
+    return ca;
+}
+
+template <typename T>
+FftArgs<T> fft_args(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, const void *src, void *dst)
+{
+    typedef typename cplx_t<T>::type T2;
+    FftArgs<T> a;
+    a.ca = ca;
+    a.fd = c->fd;
+    a.tw = reinterpret_cast<const T2 *>(sizeof(T) == 4 ? c->d_tw32.p : c->d_tw64.p);
+    a.rev = reinterpret_cast<const int *>(c->d_rev.p);
+    a.src = src;
+    a.dst = dst;
+    a.scratch = reinterpret_cast<T2 *>(pl->d_scratch.p);
+    a.tile = 0;
+    return a;
+}
+
+RecTables rec_tables(Class *c)
+{
+    RecTables rt;
+    rt.cb = (const double *)c->d_cb.p;
+    rt.lnc = (const double *)c->d_lnc.p;
+    rt.lns = (const double *)c->d_lns.p;
+    rt.w = (const double *)c->d_w.p;
+    rt.lnbin = (const double *)c->d_lnbin.p;
+    rt.sq = (const double *)c->d_sq.p;
+    rt.isq = (const double *)c->d_isq.p;
+    return rt;
+}
+
+template <typename T, int C>
+int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst,
+                    hipStream_t s)
+{
+    ConArgs<T> a;
+    a.ca = ca;
+    a.rt = rec_tables(c);
+    a.s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    a.src = src;
+    a.dst = dst;
+    const int np = a.s2 ? 2 : 4, ncol = 2 * np * C;
+    if (fwd) {
+        a.items = (const int2 *)c->d_items_fwd.p;
+        a.nitems = c->nitems_fwd;
+        a.tile = c->lt_fwd;
+        size_t lds = ((size_t)c->N * a.tile + 64 * (size_t)ncol) * sizeof(T);
+        hipLaunchKernelGGL((contract_fwd<T, C>), dim3(xcd_grid(a.nitems)), dim3(32 * C), lds, s, a);
+    } else {
+        a.items = (const int2 *)c->d_items_inv.p;
+        a.nitems = c->nitems_inv;
+        a.tile = c->kt_inv;
+        size_t lds = ((size_t)c->B * a.tile + (size_t)c->B * ncol) * sizeof(T);
+        hipLaunchKernelGGL((contract_inv<T, C>), dim3(xcd_grid(a.nitems)), dim3(32 * C), lds, s, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
+template <typename T>
+int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
+{
+    switch (pl->C) {
+    case 1: return launch_contract<T, 1>(pl, c, ca, fwd, src, dst, s);
+    case 2: return launch_contract<T, 2>(pl, c, ca, fwd, src, dst, s);
+    case 4: return launch_contract<T, 4>(pl, c, ca, fwd, src, dst, s);
+    default: return launch_contract<T, 8>(pl, c, ca, fwd, src, dst, s);
+    }
+}
+
+template <typename T>
+int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void *in, void *out, hipStream_t s)
+{
+    typedef typename cplx_t<T>::type T2;
+    const bool real = pl->kind == FFTSO3_REAL;
+    const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    const int N = c->N;
+    ChunkArgs ca = chunk_args<T>(pl, c, i0, nfc);
+    // contraction-side addressing: the real path stages complex coefficients in d_fc
+    ChunkArgs cc = ca;
+    if (real) {
+        cc.fn = reinterpret_cast<const int *>(c->d_ident.p);
+        cc.coef_off = reinterpret_cast<const long *>(c->d_ws_off.p);
+    }
+    const long nc = s2 ? (long)c->B * c->B : so3_ncoef(c->B);
+    int st;
+    const int tmix_blocks = (int)std::min<long>(4096, (nc * nfc + 255) / 256);
+    if (fwd) {
+        {
+            StageTimer t(pl, s, 0);
+            FftArgs<T> a = fft_args<T>(pl, c, ca, in, pl->d_ws.p);
+            if (s2) {
+                a.tile = c->row_tile;
+                int nunit = nfc * (N / a.tile);
+                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                if (real) hipLaunchKernelGGL((s2_fft_fwd<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((s2_fft_fwd<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            } else if (c->fused) {
+                size_t lds = (size_t)N * N * sizeof(T2);
+                if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+            } else {
+                a.tile = c->row_tile;
+                int nunit = nfc * N * (N / a.tile);
+                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_rows<T, false, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+                HIP_TRY(hipGetLastError());
+                a.tile = c->col_tile;
+                nunit = nfc * N * (N / a.tile);
+                lds = (size_t)a.tile * N * sizeof(T2);
+                hipLaunchKernelGGL((so3_fft_cols<T, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        {
+            StageTimer t(pl, s, 1);
+            if ((st = contract<T>(pl, c, cc, true, pl->d_ws.p, real ? pl->d_fc.p : out, s))) return st;
+        }
+        if (real) {
+            StageTimer t(pl, s, 2);
+            if (s2) hipLaunchKernelGGL((s2_tmix<T, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_fc.p, out, nc);
+            else hipLaunchKernelGGL((so3_tmix<T, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_fc.p, out, nc);
+            HIP_TRY(hipGetLastError());
+        }
+    } else {
+        const void *csrc = in;
+        if (real) {
+            StageTimer t(pl, s, 3);
+            if (s2) hipLaunchKernelGGL((s2_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
+            else hipLaunchKernelGGL((so3_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
+            HIP_TRY(hipGetLastError());
+            csrc = pl->d_fc.p;
+        }
+        {
+            StageTimer t(pl, s, 4);
+            if ((st = contract<T>(pl, c, cc, false, csrc, pl->d_ws.p, s))) return st;
+        }
+        {
+            StageTimer t(pl, s, 5);
+            FftArgs<T> a = fft_args<T>(pl, c, ca, pl->d_ws.p, out);
+            if (s2) {
+                a.tile = c->row_tile;
+                int nunit = nfc * (N / a.tile);
+                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                if (real) hipLaunchKernelGGL((s2_fft_inv<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((s2_fft_inv<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            } else if (c->fused) {
+                size_t lds = (size_t)N * N * sizeof(T2);
+                if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+            } else {
+                a.tile = c->col_tile;
+                int nunit = nfc * N * (N / a.tile);
+                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                hipLaunchKernelGGL((so3_fft_cols<T, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+                HIP_TRY(hipGetLastError());
+                a.tile = c->row_tile;
+                nunit = nfc * N * (N / a.tile);
+                lds = (size_t)a.tile * N * sizeof(T2);
+                if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_rows<T, false, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    return FFTSO3_OK;
+}
+
+template <typename T>
+int run(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStream_t s)
+{
+    int st;
+    for (auto *c : pl->classes) {
+        const int n = (int)c->fns.size();
+        for (int i0 = 0; i0 < n; i0 += pl->C)
+            if ((st = run_chunk<T>(pl, c, i0, std::min(pl->C, n - i0), fwd, in, out, s))) return st;
+    }
+    return FFTSO3_OK;
+}
+
+bool kernel_attrs_done = false;
+int set_kernel_attrs()
+{
+    if (kernel_attrs_done) return FFTSO3_OK;
+#define ALLOW_LDS(k) HIP_TRY(hipFuncSetAttribute((const void *)(k), hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds))
+    ALLOW_LDS((so3_fft_fwd_slice<float, true>));
+    ALLOW_LDS((so3_fft_fwd_slice<float, false>));
+    ALLOW_LDS((so3_fft_fwd_slice<double, true>));
+    ALLOW_LDS((so3_fft_fwd_slice<double, false>));
+    ALLOW_LDS((so3_fft_inv_slice<float, true>));
+    ALLOW_LDS((so3_fft_inv_slice<float, false>));
+    ALLOW_LDS((so3_fft_inv_slice<double, true>));
+    ALLOW_LDS((so3_fft_inv_slice<double, false>));
+    ALLOW_LDS((so3_fft_rows<float, true, 0>));
+    ALLOW_LDS((so3_fft_rows<float, false, 0>));
+    ALLOW_LDS((so3_fft_rows<float, true, 1>));
+    ALLOW_LDS((so3_fft_rows<float, false, 1>));
+    ALLOW_LDS((so3_fft_rows<double, true, 0>));
+    ALLOW_LDS((so3_fft_rows<double, false, 0>));
+    ALLOW_LDS((so3_fft_rows<double, true, 1>));
+    ALLOW_LDS((so3_fft_rows<double, false, 1>));
+    ALLOW_LDS((so3_fft_cols<float, 0>));
+    ALLOW_LDS((so3_fft_cols<float, 1>));
+    ALLOW_LDS((so3_fft_cols<double, 0>));
+    ALLOW_LDS((so3_fft_cols<double, 1>));
+    ALLOW_LDS((s2_fft_fwd<float, true>));
+    ALLOW_LDS((s2_fft_fwd<float, false>));
+    ALLOW_LDS((s2_fft_fwd<double, true>));
+    ALLOW_LDS((s2_fft_fwd<double, false>));
+    ALLOW_LDS((s2_fft_inv<float, true>));
+    ALLOW_LDS((s2_fft_inv<float, false>));
+    ALLOW_LDS((s2_fft_inv<double, true>));
+    ALLOW_LDS((s2_fft_inv<double, false>));
+    ALLOW_LDS((contract_fwd<float, 1>));
+    ALLOW_LDS((contract_fwd<float, 2>));
+    ALLOW_LDS((contract_fwd<float, 4>));
+    ALLOW_LDS((contract_fwd<float, 8>));
+    ALLOW_LDS((contract_fwd<double, 1>));
+    ALLOW_LDS((contract_fwd<double, 2>));
+    ALLOW_LDS((contract_fwd<double, 4>));
+    ALLOW_LDS((contract_fwd<double, 8>));
+    ALLOW_LDS((contract_inv<float, 1>));
+    ALLOW_LDS((contract_inv<float, 2>));
+    ALLOW_LDS((contract_inv<float, 4>));
+    ALLOW_LDS((contract_inv<float, 8>));
+    ALLOW_LDS((contract_inv<double, 1>));
+    ALLOW_LDS((contract_inv<double, 2>));
+    ALLOW_LDS((contract_inv<double, 4>));
+    ALLOW_LDS((contract_inv<double, 8>));
+#undef ALLOW_LDS
+    kernel_attrs_done = true;
+    return FFTSO3_OK;
+}
+
+int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int kind, int precision)
+{
+    if (!out) return fail(FFTSO3_EINVAL, "plan pointer is NULL");
+    *out = nullptr;
+    if (domain != FFTSO3_DOMAIN_SO3 && domain != FFTSO3_DOMAIN_S2) return fail(FFTSO3_EINVAL, "unknown domain %d", domain);
+    if (kind != FFTSO3_COMPLEX && kind != FFTSO3_REAL) return fail(FFTSO3_EINVAL, "unknown kind %d", kind);
+    if (precision < FFTSO3_FP64 || precision > FFTSO3_BF16X6) return fail(FFTSO3_EINVAL, "unknown precision %d", precision);
+    if (n_fn <= 0) return fail(FFTSO3_EINVAL, "n_fn must be positive (got %d)", n_fn);
+    const int maxB = domain == FFTSO3_DOMAIN_SO3 ? 256 : 1024;
+    for (int i = 0; i < n_fn; i++)
+        if (lmax[i] < 0 || lmax[i] + 1 > maxB)
+            return fail(FFTSO3_EINVAL, "l_max[%d] = %d outside [0, %d]", i, lmax[i], maxB - 1);
+    int st;
+    if ((st = set_kernel_attrs())) return st;
+    fftso3_plan_s *pl = new (std::nothrow) fftso3_plan_s();
+    if (!pl) return fail(FFTSO3_ENOMEM, "host allocation failed");
+    pl->domain = domain;
+    pl->kind = kind;
+    pl->prec = precision;
+    pl->n_fn = n_fn;
+    pl->esz = precision == FFTSO3_FP64 ? 16 : 8;
+    pl->Bs.resize(n_fn);
+    pl->grid_off.resize(n_fn);
+    pl->coef_off.resize(n_fn);
+    long go = 0, co = 0;
+    for (int i = 0; i < n_fn; i++) {
+        long B = lmax[i] + 1, N = 2 * B;
+        pl->Bs[i] = (int)B;
+        pl->grid_off[i] = go;
+        pl->coef_off[i] = co;
+        go += domain == FFTSO3_DOMAIN_SO3 ? N * N * N : N * N;
+        co += domain == FFTSO3_DOMAIN_SO3 ? so3_ncoef((int)B) : B * B;
+    }
+    pl->grid_total = go;
+    pl->coef_total = co;
+    int maxfn = 1;
+    {
+        std::map<int, int> cnt;
+        for (int B : pl->Bs) maxfn = std::max(maxfn, ++cnt[B]);
+    }
+    pl->C = 1;
+    while (pl->C < 8 && pl->C < maxfn) pl->C *= 2;
+    if ((st = pl->d_grid_off.upload(pl->grid_off)) || (st = pl->d_coef_off.upload(pl->coef_off)) ||
+        (st = finalize_plan(pl))) {
+        delete pl;
+        return st;
+    }
+    *out = pl;
+    return FFTSO3_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+int fftso3_plan_create(fftso3_plan_t *plan, int domain, int l_max, int n_fn, int kind, int precision, unsigned flags)
+{
+    (void)flags;
+    if (n_fn <= 0) return fail(FFTSO3_EINVAL, "n_fn must be positive (got %d)", n_fn);
+    std::vector<int> lm((size_t)n_fn, l_max);
+    return create_common(plan, domain, lm.data(), n_fn, kind, precision);
+}
+
+int fftso3_plan_create_uneven(fftso3_plan_t *plan, int domain, const int *l_max, int n_fn, int kind, int precision,
+                              unsigned flags)
+{
+    (void)flags;
+    if (!l_max) return fail(FFTSO3_EINVAL, "l_max array is NULL");
+    return create_common(plan, domain, l_max, n_fn, kind, precision);
+}
+
+int fftso3_plan_destroy(fftso3_plan_t plan)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    delete plan;
+    return FFTSO3_OK;
+}
+
+int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (chunk != 1 && chunk != 2 && chunk != 4 && chunk != 8)
+        return fail(FFTSO3_EINVAL, "chunk must be 1, 2, 4 or 8 (got %d)", chunk);
+    if (chunk == plan->C) return FFTSO3_OK;
+    plan->C = chunk;
+    return finalize_plan(plan);
+}
+
+int fftso3_plan_get_chunk(fftso3_plan_t plan, int *chunk)
+{
+    if (!plan || !chunk) return fail(FFTSO3_EINVAL, "NULL argument");
+    *chunk = plan->C;
+    return FFTSO3_OK;
+}
+
+long fftso3_plan_grid_offset(fftso3_plan_t plan, int fn)
+{
+    if (!plan || fn < 0 || fn >= plan->n_fn) return -1;
+    return plan->grid_off[fn];
+}
+long fftso3_plan_coef_offset(fftso3_plan_t plan, int fn)
+{
+    if (!plan || fn < 0 || fn >= plan->n_fn) return -1;
+    return plan->coef_off[fn];
+}
+long fftso3_plan_grid_elems(fftso3_plan_t plan) { return plan ? plan->grid_total : -1; }
+long fftso3_plan_coef_elems(fftso3_plan_t plan) { return plan ? plan->coef_total : -1; }
+long fftso3_plan_workspace_bytes(fftso3_plan_t plan)
+{
+    if (!plan) return -1;
+    return (long)(plan->ws_bytes + plan->scratch_bytes + plan->fc_bytes);
+}
+
+static int run_dispatch(fftso3_plan_t plan, bool fwd, const void *in, void *out, void *stream)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (!in || !out) return fail(FFTSO3_EINVAL, "NULL device pointer");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (plan->prec) {
+    case FFTSO3_FP64: return run<double>(plan, fwd, in, out, s);
+    case FFTSO3_FP32: return run<float>(plan, fwd, in, out, s);
+    default: return fail(FFTSO3_EUNSUPPORTED, "precision %d not implemented", plan->prec);
+    }
+}
+
+int fftso3_forward(fftso3_plan_t plan, const void *d_grid, void *d_coef, void *stream)
+{
+    return run_dispatch(plan, true, d_grid, d_coef, stream);
+}
+
+int fftso3_inverse(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream)
+{
+    return run_dispatch(plan, false, d_coef, d_grid, stream);
+}
+
+const char *fftso3_status_string(int status)
+{
+    switch (status) {
+    case FFTSO3_OK: return "FFTSO3_OK";
+    case FFTSO3_EINVAL: return "FFTSO3_EINVAL: invalid argument";
+    case FFTSO3_ENOMEM: return "FFTSO3_ENOMEM: allocation failed";
+    case FFTSO3_EHIP: return "FFTSO3_EHIP: HIP runtime error";
+    case FFTSO3_EUNSUPPORTED: return "FFTSO3_EUNSUPPORTED: not supported";
+    default: return "unknown status";
+    }
+}
+
+const char *fftso3_last_error(void) { return g_last_error.c_str(); }
+
+int fftso3_timing_enable(fftso3_plan_t plan, int on)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    plan->timing = on != 0;
+    plan->ev_used.clear();
+    for (int i = 0; i < kNumStages; i++) { plan->acc_ms[i] = 0; plan->acc_n[i] = 0; }
+    return FFTSO3_OK;
+}
+
+int fftso3_timing_read(fftso3_plan_t plan, double *ms, long *count)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    for (auto &u : plan->ev_used) {
+        auto &e = plan->ev_pool[u.second];
+        HIP_TRY(hipEventSynchronize(e.second));
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, e.first, e.second));
+        plan->acc_ms[u.first] += t;
+        plan->acc_n[u.first] += 1;
+    }
+    plan->ev_used.clear();
+    for (int i = 0; i < kNumStages; i++) {
+        if (ms) ms[i] = plan->acc_ms[i];
+        if (count) count[i] = plan->acc_n[i];
+        plan->acc_ms[i] = 0;
+        plan->acc_n[i] = 0;
+    }
+    return FFTSO3_OK;
+}
+
+long fftso3_so3_ncoef(int l_max) { return l_max < 0 ? -1 : so3_ncoef(l_max + 1); }
+long fftso3_s2_ncoef(int l_max) { return l_max < 0 ? -1 : (long)(l_max + 1) * (l_max + 1); }
+
+}  // extern "C"

@@ -1,0 +1,661 @@
+// kernels.hpp -- HIP kernels (gfx950) for the SO(3) and S^2 harmonic transforms.
+//
+// Pipeline per chunk of C functions of one bandlimit B (N = 2B):
+//   forward:  grid --fft kernels--> spectrum workspace S --contraction--> coefficients
+//   inverse:  coefficients --contraction^T--> workspace G --fft kernels--> grid
+// The workspace is group-major: S[g][k][fi][p] (complex), g the canonical (m0,n0) group,
+// k the beta index, fi the function within the chunk, p the symmetry pair (4 for SO(3),
+// 2 for S^2).  The contraction of one group is a small real-times-complex GEMM whose
+// Wigner-d (or Legendre) matrix is regenerated on the fly in FP64 by the three-term recursion
+// in l and kept in LDS; nothing of the d-table ever goes to HBM.
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+
+#define XCD_COUNT 8
+
+// Map blockIdx.x so that consecutive work units land on the same XCD (each XCD has its own
+// L2; the hardware deals workgroups round-robin over the 8 XCDs).
+__device__ __forceinline__ int xcd_unit(int nunits)
+{
+    int q = (nunits + XCD_COUNT - 1) / XCD_COUNT;
+    int b = blockIdx.x;
+    return (b % XCD_COUNT) * q + b / XCD_COUNT;
+}
+static inline int xcd_grid(int nunits) { return ((nunits + XCD_COUNT - 1) / XCD_COUNT) * XCD_COUNT; }
+
+// ---------------------------------------------------------------------------------------------
+// Per-chunk addressing shared by all kernels
+// ---------------------------------------------------------------------------------------------
+struct ChunkArgs {
+    int B, N;
+    int C;        // workspace stride in functions (chunk capacity)
+    int nfc;      // functions in this chunk (<= C)
+    const int *fn;          // chunk-local index -> global function index (device)
+    const long *grid_off;   // global function -> grid offset (elements)
+    const long *coef_off;   // global function -> coefficient offset (elements)
+};
+
+// ---------------------------------------------------------------------------------------------
+// SO(3) FFT kernels
+// ---------------------------------------------------------------------------------------------
+template <typename T> struct FftArgs {
+    ChunkArgs ca;
+    FftDesc fd;
+    const typename cplx_t<T>::type *tw;  // W_N^t, t < N
+    const int *rev;                      // frequency -> position in the DIF output
+    const void *src;                     // grid (forward) or workspace (inverse)
+    void *dst;                           // workspace (forward) or grid (inverse)
+    typename cplx_t<T>::type *scratch;   // two-pass fallback: [fi][k][j1][v] complex
+    int tile;                            // two-pass fallback: rows (or columns) per workgroup
+};
+
+// Fused forward: one (fi, k) slice per workgroup, whole N x N slice in LDS.
+template <typename T, bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft_fwd_slice(FftArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    const ChunkArgs &ca = a.ca;
+    const int N = ca.N, B = ca.B, nslice = ca.nfc * N;
+    const int unit = xcd_unit(nslice);
+    if (unit >= nslice) return;
+    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+    const int fn = ca.fn[fi];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long base = ca.grid_off[fn] + (long)k * N * N;
+    if (REAL) {
+        const T *g = reinterpret_cast<const T *>(a.src) + base;
+        for (int e = tid; e < N * N; e += nthr) buf[e] = cmk<T2>(g[e], (T)0);
+    } else {
+        const T2 *g = reinterpret_cast<const T2 *>(a.src) + base;
+        for (int e = tid; e < N * N; e += nthr) buf[e] = g[e];
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, N, true, a.tw, tid, nthr);   // over j2 (gamma)
+    fft_tile(buf, a.fd, N, 1, N, false, a.tw, tid, nthr);  // over j1 (alpha)
+    // emit S_k(m,n) = X(-m mod N, -n mod N) to its group slot
+    T2 *S = reinterpret_cast<T2 *>(a.dst);
+    const int nq = 4 * B * B;
+    for (int q = tid; q < nq; q += nthr) {
+        int g = q >> 2, p = q & 3, m0, n0, m, n;
+        so3_group(g, m0, n0);
+        so3_pair(m0, n0, p, m, n);
+        int u = a.rev[wrapN(-m, N)], v = a.rev[wrapN(-n, N)];
+        S[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[u * N + v];
+    }
+}
+
+// Fused inverse: gather G_k(m,n) into bins (m mod N, n mod N), 2-D DFT, store the slice.
+template <typename T, bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft_inv_slice(FftArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    const ChunkArgs &ca = a.ca;
+    const int N = ca.N, B = ca.B, nslice = ca.nfc * N;
+    const int unit = xcd_unit(nslice);
+    if (unit >= nslice) return;
+    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+    const int fn = ca.fn[fi];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const T2 *G = reinterpret_cast<const T2 *>(a.src);
+    for (int e = tid; e < N * N; e += nthr) {
+        int u = e / N, v = e - u * N;
+        T2 val = cmk<T2>((T)0, (T)0);
+        if (u != B && v != B) {
+            int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
+            so3_slot(m, n, g, p);
+            val = G[(((long)g * N + k) * ca.C + fi) * 4 + p];
+        }
+        buf[e] = val;
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, N, true, a.tw, tid, nthr);
+    fft_tile(buf, a.fd, N, 1, N, false, a.tw, tid, nthr);
+    const long base = ca.grid_off[fn] + (long)k * N * N;
+    for (int e = tid; e < N * N; e += nthr) {
+        int j1 = e / N, j2 = e - j1 * N;
+        T2 val = buf[a.rev[j1] * N + a.rev[j2]];
+        if (REAL) reinterpret_cast<T *>(a.dst)[base + e] = val.x;
+        else reinterpret_cast<T2 *>(a.dst)[base + e] = val;
+    }
+}
+
+// Two-pass fallback for slices larger than LDS.
+// Rows pass: `tile` rows of one (fi,k) slice; DFT over the contiguous axis; natural order out.
+// MODE 0: grid -> scratch (forward first pass), MODE 1: scratch -> grid (inverse second pass).
+template <typename T, bool REAL, int MODE>
+__global__ void __launch_bounds__(1024) so3_fft_rows(FftArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    const ChunkArgs &ca = a.ca;
+    const int N = ca.N, R = a.tile, nblk = N / R, nunit = ca.nfc * N * nblk;
+    const int unit = xcd_unit(nunit);
+    if (unit >= nunit) return;
+    const int rb = unit % nblk, k = (unit / nblk) % N, fi = unit / (nblk * N);
+    const int fn = ca.fn[fi];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long gbase = ca.grid_off[fn] + ((long)k * N + (long)rb * R) * N;
+    const long sbase = (((long)fi * N + k) * N + (long)rb * R) * N;
+    if (MODE == 0) {
+        if (REAL) {
+            const T *g = reinterpret_cast<const T *>(a.src) + gbase;
+            for (int e = tid; e < R * N; e += nthr) buf[e] = cmk<T2>(g[e], (T)0);
+        } else {
+            const T2 *g = reinterpret_cast<const T2 *>(a.src) + gbase;
+            for (int e = tid; e < R * N; e += nthr) buf[e] = g[e];
+        }
+    } else {
+        for (int e = tid; e < R * N; e += nthr) buf[e] = a.scratch[sbase + e];
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, R, true, a.tw, tid, nthr);
+    for (int e = tid; e < R * N; e += nthr) {
+        int r = e / N, v = e - r * N;
+        T2 val = buf[r * N + a.rev[v]];
+        if (MODE == 0) a.scratch[sbase + e] = val;
+        else if (REAL) reinterpret_cast<T *>(a.dst)[gbase + e] = val.x;
+        else reinterpret_cast<T2 *>(a.dst)[gbase + e] = val;
+    }
+}
+
+// Columns pass over `tile` columns of one (fi,k) slice.
+// MODE 0 (forward): scratch -> DFT over j1 -> emit to the group-major workspace.
+// MODE 1 (inverse): gather from the workspace -> DFT over the m bins -> scratch.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);  // [row][tc]
+    const ChunkArgs &ca = a.ca;
+    const int N = ca.N, B = ca.B, TC = a.tile, nblk = N / TC, nunit = ca.nfc * N * nblk;
+    const int unit = xcd_unit(nunit);
+    if (unit >= nunit) return;
+    const int cb = unit % nblk, k = (unit / nblk) % N, fi = unit / (nblk * N);
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long sbase = ((long)fi * N + k) * N * N;
+    T2 *W = reinterpret_cast<T2 *>(MODE == 0 ? a.dst : const_cast<void *>(a.src));
+    if (MODE == 0) {
+        for (int e = tid; e < N * TC; e += nthr) {
+            int r = e / TC, c = e - r * TC;
+            buf[e] = a.scratch[sbase + (long)r * N + cb * TC + c];
+        }
+    } else {
+        for (int e = tid; e < N * TC; e += nthr) {
+            int u = e / TC, c = e - u * TC, v = cb * TC + c;
+            T2 val = cmk<T2>((T)0, (T)0);
+            if (u != B && v != B) {
+                int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
+                so3_slot(m, n, g, p);
+                val = W[(((long)g * N + k) * ca.C + fi) * 4 + p];
+            }
+            buf[e] = val;
+        }
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, TC, 1, TC, false, a.tw, tid, nthr);
+    if (MODE == 0) {
+        for (int e = tid; e < N * TC; e += nthr) {
+            int u = e / TC, c = e - u * TC, v = cb * TC + c;
+            if (u == B || v == B) continue;
+            int m = wrapN(-u, N), n = wrapN(-v, N);
+            m = m < B ? m : m - N;
+            n = n < B ? n : n - N;
+            int g, p;
+            so3_slot(m, n, g, p);
+            // duplicated slots of orbits of size 1 or 2 are left unwritten: the contraction's
+            // columns are independent and it never stores results for invalid pairs
+            W[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[a.rev[u] * TC + c];
+        }
+    } else {
+        for (int e = tid; e < N * TC; e += nthr) {
+            int j1 = e / TC, c = e - j1 * TC;
+            a.scratch[sbase + (long)j1 * N + cb * TC + c] = buf[a.rev[j1] * TC + c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// S^2 FFT kernels: `tile` rings of one function per workgroup
+// ---------------------------------------------------------------------------------------------
+template <typename T, bool REAL>
+__global__ void __launch_bounds__(1024) s2_fft_fwd(FftArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    const ChunkArgs &ca = a.ca;
+    const int N = ca.N, B = ca.B, R = a.tile, nblk = N / R, nunit = ca.nfc * nblk;
+    const int unit = xcd_unit(nunit);
+    if (unit >= nunit) return;
+    const int rb = unit % nblk, fi = unit / nblk;
+    const int fn = ca.fn[fi];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const long gbase = ca.grid_off[fn] + (long)rb * R * N;
+    if (REAL) {
+        const T *g = reinterpret_cast<const T *>(a.src) + gbase;
+        for (int e = tid; e < R * N; e += nthr) buf[e] = cmk<T2>(g[e], (T)0);
+    } else {
+        const T2 *g = reinterpret_cast<const T2 *>(a.src) + gbase;
+        for (int e = tid; e < R * N; e += nthr) buf[e] = g[e];
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, R, true, a.tw, tid, nthr);
+    // S_k(m) = X(m mod N) (src/fdcl_FFTS2.cpp:159-166); slot (|m|, m<0)
+    T2 *S = reinterpret_cast<T2 *>(a.dst);
+    const int nq = 2 * B;
+    for (int e = tid; e < R * nq; e += nthr) {
+        int r = e / nq, q = e - r * nq, g = q >> 1, p = q & 1;
+        int m = p ? -g : g;
+        int k = rb * R + r;
+        S[(((long)g * N + k) * ca.C + fi) * 2 + p] = buf[r * N + a.rev[wrapN(m, N)]];
+    }
+}
+
+template <typename T, bool REAL>
+__global__ void __launch_bounds__(1024) s2_fft_inv(FftArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    const ChunkArgs &ca = a.ca;
+    const int N = ca.N, B = ca.B, R = a.tile, nblk = N / R, nunit = ca.nfc * nblk;
+    const int unit = xcd_unit(nunit);
+    if (unit >= nunit) return;
+    const int rb = unit % nblk, fi = unit / nblk;
+    const int fn = ca.fn[fi];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const T2 *G = reinterpret_cast<const T2 *>(a.src);
+    // G_k(m) at bin (-m mod N)
+    for (int e = tid; e < R * N; e += nthr) {
+        int r = e / N, u = e - r * N, k = rb * R + r;
+        T2 val = cmk<T2>((T)0, (T)0);
+        if (u != B) {
+            int mneg = u < B ? u : u - N;   // bin u holds m = -mneg
+            int m = -mneg;
+            int g = iabs(m), p = m < 0;
+            val = G[(((long)g * N + k) * ca.C + fi) * 2 + p];
+        }
+        buf[e] = val;
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, R, true, a.tw, tid, nthr);
+    const long gbase = ca.grid_off[fn] + (long)rb * R * N;
+    for (int e = tid; e < R * N; e += nthr) {
+        int r = e / N, j = e - r * N;
+        T2 val = buf[r * N + a.rev[j]];
+        if (REAL) reinterpret_cast<T *>(a.dst)[gbase + e] = val.x;
+        else reinterpret_cast<T2 *>(a.dst)[gbase + e] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 recursions (one lane per beta_k), shared by the contraction kernels
+// ---------------------------------------------------------------------------------------------
+struct RecTables {
+    const double *cb;      // cos(beta_k)                        [N]
+    const double *lnc;     // ln cos(beta_k/2)                   [N]
+    const double *lns;     // ln sin(beta_k/2)                   [N]
+    const double *w;       // quadrature weight w_k              [N]
+    const double *lnbin;   // SO(3): 0.5 ln C(2m0, m0+n0) per g  [B^2]; S^2: seed log per m0 [B]
+    const double *sq;      // SO(3): sqrt(l^2 - m^2)             [B*B] (l-major); S^2: fact table
+    const double *isq;     // SO(3): 1/sqrt(l^2 - m^2);          S^2: 1/fact
+};
+
+// Wigner d^l_{m0 n0}(beta_k) for l = m0, m0+1, ...: seed d^{m0}_{m0 n0} =
+// sqrt(C(2m0, m0+n0)) cos(b/2)^(m0+n0) (-sin(b/2))^(m0-n0), then the three-term recursion (64)
+// of src/fdcl_FFTSO3.cpp:154-161:
+//   d^l = [ (l(2l-1) cb - (2l-1) m n/(l-1)) d^{l-1} - l/(l-1) sqrt(((l-1)^2-m^2)((l-1)^2-n^2)) d^{l-2} ]
+//         / sqrt((l^2-m^2)(l^2-n^2)).
+struct WignerRec {
+    int B, m0, n0, an0;
+    double cb, d1, d2;  // d1 = d^{l-1}, d2 = d^{l-2}
+    const double *sq, *isq;
+    __device__ void init(const RecTables &t, int B_, int g, int m0_, int n0_, int k)
+    {
+        B = B_; m0 = m0_; n0 = n0_; an0 = iabs(n0_);
+        cb = t.cb[k];
+        sq = t.sq; isq = t.isq;
+        int a = m0 + n0, b = m0 - n0;
+        double x = t.lnbin[g] + (a ? a * t.lnc[k] : 0.0) + (b ? b * t.lns[k] : 0.0);
+        double s = exp(x);
+        d1 = (b & 1) ? -s : s;
+        d2 = 0.0;
+    }
+    // value at l = m0 is d1 after init; advance to l (l > m0) and return d^l
+    __device__ __forceinline__ double step(int l)
+    {
+        double v;
+        if (l == 1) {
+            v = cb * d1;
+        } else {
+            double dl = (double)l;
+            double ir = isq[l * B + m0] * isq[l * B + an0];
+            double inv_lm1 = 1.0 / (dl - 1.0);
+            double A = dl * (2.0 * dl - 1.0) * ir;
+            double Bc = (2.0 * dl - 1.0) * (double)(m0 * n0) * inv_lm1 * ir;
+            double Cc = dl * inv_lm1 * sq[(l - 1) * B + m0] * sq[(l - 1) * B + an0] * ir;
+            v = (A * cb - Bc) * d1 - Cc * d2;
+        }
+        d2 = d1;
+        d1 = v;
+        return v;
+    }
+};
+
+// Normalised associated Legendre nP^l_{m0}(cos theta_k) (src/fdcl_FFTS2.cpp:48-100, NR 6.7):
+//   nP(m,m) = (-1)^m sqrt((2m+1)/(4 pi) prod_{i<=m} (2i-1)/(2i)) sin^m(theta)
+//   nP(l,m) = fact_l (x nP(l-1,m) - nP(l-2,m)/fact_{l-1}),  fact_l = sqrt((4l^2-1)/(l^2-m^2))
+struct LegendreRec {
+    int B, m0;
+    double x, d1, d2;
+    const double *fact, *ifact;
+    __device__ void init(const RecTables &t, int B_, int g, int k)
+    {
+        B = B_; m0 = g;
+        x = t.cb[k];
+        fact = t.sq; ifact = t.isq;
+        double s = exp(t.lnbin[g] + (m0 ? m0 * t.lns[k] : 0.0));
+        d1 = (m0 & 1) ? -s : s;
+        d2 = 0.0;
+    }
+    __device__ __forceinline__ double step(int l)
+    {
+        double f = fact[l * B + m0];
+        double v = f * (x * d1 - (l - 2 >= m0 ? d2 * ifact[(l - 1) * B + m0] : 0.0));
+        d2 = d1;
+        d1 = v;
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Contraction kernels (vector FMA path; FP32 or FP64 arithmetic)
+// ---------------------------------------------------------------------------------------------
+template <typename T> struct ConArgs {
+    ChunkArgs ca;
+    RecTables rt;
+    const int2 *items;    // forward: (g, l_start); inverse: (g, k_start)
+    int nitems;
+    int tile;             // forward: rows per item (LT); inverse: k per item (KT)
+    const void *src;      // forward: workspace S; inverse: coefficients
+    void *dst;            // forward: coefficients; inverse: workspace G
+    int s2;               // 1 for S^2 (two pairs, Legendre recursion)
+};
+
+// Forward: F^l_{pair} = s_p sum_k w_k d^l_{m0 n0}(beta_k) S_k(pair), l in [l_start, l_start+LT).
+// Block = 32*C threads: 8C real columns (fi, p, re/im) x 4 row groups of 4 rows.  (S^2 uses
+// 4C real columns and 8 row groups.)
+template <typename T, int C>
+__global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    constexpr int NT = 32 * C;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, LT = a.tile;
+    const int unit = xcd_unit(a.nitems);
+    if (unit >= a.nitems) return;
+    const int2 it = a.items[unit];
+    const int g = it.x, l0 = it.y;
+    const int np = a.s2 ? 2 : 4;
+    const int ncol = 2 * np * C;             // real columns
+    int m0, n0;
+    if (a.s2) { m0 = g; n0 = 0; } else so3_group(g, m0, n0);
+    const int lend = min(B, l0 + LT);
+    T *D = reinterpret_cast<T *>(smem_raw);          // [k][LT]
+    T *Ss = D + (size_t)N * LT;                      // [KC][ncol]
+    const int tid = threadIdx.x;
+    // phase 1: FP64 recursion, one lane per k; D[k][l-l0] = w_k d^l(beta_k)
+    for (int k = tid; k < N; k += NT) {
+        double wk = a.rt.w[k];
+        double v;
+        if (a.s2) {
+            LegendreRec r;
+            r.init(a.rt, B, g, k);
+            v = r.d1;
+            for (int l = m0; l < lend; l++) {
+                if (l > m0) v = r.step(l);
+                if (l >= l0) D[(size_t)k * LT + (l - l0)] = (T)(wk * v);
+            }
+        } else {
+            WignerRec r;
+            r.init(a.rt, B, g, m0, n0, k);
+            v = r.d1;
+            for (int l = m0; l < lend; l++) {
+                if (l > m0) v = r.step(l);
+                if (l >= l0) D[(size_t)k * LT + (l - l0)] = (T)(wk * v);
+            }
+        }
+        for (int l = lend; l < l0 + LT; l++) D[(size_t)k * LT + (l - l0)] = (T)0;
+    }
+    // phase 2: GEMM over k in chunks staged in LDS
+    const int rows_per_thread = 4;
+    const int ngroups = NT / ncol;
+    const int col = tid % ncol, rg = tid / ncol;
+    const T *S = reinterpret_cast<const T *>(a.src) + (size_t)g * N * C * np * 2;
+    constexpr int KC = 64;
+    T acc[rows_per_thread] = {};
+    for (int kc = 0; kc < N; kc += KC) {
+        const int kn = min(KC, N - kc);
+        __syncthreads();
+        for (int e = tid; e < kn * ncol; e += NT) Ss[e] = S[(size_t)kc * ncol + e];
+        __syncthreads();
+        if (rg < ngroups) {
+            for (int kk = 0; kk < kn; kk++) {
+                T s = Ss[kk * ncol + col];
+                const T *dk = D + (size_t)(kc + kk) * LT + rg * rows_per_thread;
+#pragma unroll
+                for (int r = 0; r < rows_per_thread; r++) acc[r] += dk[r] * s;
+            }
+        }
+    }
+    if (rg >= ngroups) return;
+    const int ri = col & 1, p = (col >> 1) % np, fi = col / (2 * np);
+    if (fi >= ca.nfc) return;
+    int m, n, sp;
+    bool valid;
+    if (a.s2) {
+        m = p ? -m0 : m0; n = 0;
+        sp = p ? parity_sign(m0) : 1;
+        valid = (p == 0) || (m0 > 0);
+    } else {
+        so3_pair(m0, n0, p, m, n);
+        sp = so3_pair_sign(m0, n0, p);
+        valid = so3_pair_valid(m0, n0, p);
+    }
+    if (!valid) return;
+    T *out = reinterpret_cast<T *>(a.dst) + 2 * ca.coef_off[ca.fn[fi]];
+#pragma unroll
+    for (int r = 0; r < rows_per_thread; r++) {
+        int l = l0 + rg * rows_per_thread + r;
+        if (l < lend) {
+            long idx = a.s2 ? s2_idx(l, m) : so3_idx(l, m, n);
+            out[2 * idx + ri] = (T)sp * acc[r];
+        }
+    }
+}
+
+// Inverse: G_k(pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) F^l(pair), c_l = 2l+1 (SO(3)) or 1 (S^2),
+// k in [k_start, k_start+KT).  Block = 32*C threads: 8C real columns x 4 k groups.
+template <typename T, int C>
+__global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
+{
+    constexpr int NT = 32 * C;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, KT = a.tile;
+    const int unit = xcd_unit(a.nitems);
+    if (unit >= a.nitems) return;
+    const int2 it = a.items[unit];
+    const int g = it.x, k0 = it.y;
+    const int np = a.s2 ? 2 : 4;
+    const int ncol = 2 * np * C;
+    int m0, n0;
+    if (a.s2) { m0 = g; n0 = 0; } else so3_group(g, m0, n0);
+    const int M = B - m0;
+    const int kn = min(KT, N - k0);
+    T *D = reinterpret_cast<T *>(smem_raw);     // [l-m0][KT]
+    T *Fv = D + (size_t)B * KT;                 // [l-m0][ncol]
+    const int tid = threadIdx.x;
+    for (int kk = tid; kk < KT; kk += NT) {
+        int k = k0 + kk;
+        if (kk >= kn) {
+            for (int l = m0; l < B; l++) D[(size_t)(l - m0) * KT + kk] = (T)0;
+            continue;
+        }
+        if (a.s2) {
+            LegendreRec r;
+            r.init(a.rt, B, g, k);
+            double v = r.d1;
+            for (int l = m0; l < B; l++) {
+                if (l > m0) v = r.step(l);
+                D[(size_t)(l - m0) * KT + kk] = (T)v;
+            }
+        } else {
+            WignerRec r;
+            r.init(a.rt, B, g, m0, n0, k);
+            double v = r.d1;
+            for (int l = m0; l < B; l++) {
+                if (l > m0) v = r.step(l);
+                D[(size_t)(l - m0) * KT + kk] = (T)((double)(2 * l + 1) * v);
+            }
+        }
+    }
+    // stage the coefficients of the group's pairs (sign folded in)
+    for (int e = tid; e < M * ncol; e += NT) {
+        int lr = e / ncol, col = e - lr * ncol, l = m0 + lr;
+        int ri = col & 1, p = (col >> 1) % np, fi = col / (2 * np);
+        T val = (T)0;
+        if (fi < ca.nfc) {
+            int m, n, sp;
+            if (a.s2) { m = p ? -m0 : m0; n = 0; sp = p ? parity_sign(m0) : 1; }
+            else { so3_pair(m0, n0, p, m, n); sp = so3_pair_sign(m0, n0, p); }
+            const T *F = reinterpret_cast<const T *>(a.src) + 2 * ca.coef_off[ca.fn[fi]];
+            long idx = a.s2 ? s2_idx(l, m) : so3_idx(l, m, n);
+            val = (T)sp * F[2 * idx + ri];
+        }
+        Fv[e] = val;
+    }
+    __syncthreads();
+    const int ngroups = NT / ncol;
+    const int col = tid % ncol, kg = tid / ncol;
+    if (kg >= ngroups) return;
+    T *G = reinterpret_cast<T *>(a.dst) + (size_t)g * N * C * np * 2;
+    for (int kb = kg * 4; kb < kn; kb += ngroups * 4) {
+        T acc[4] = {};
+        for (int lr = 0; lr < M; lr++) {
+            T f = Fv[lr * ncol + col];
+            const T *dl = D + (size_t)lr * KT + kb;
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] += dl[j] * f;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (kb + j < kn) G[(size_t)(k0 + kb + j) * ncol + col] = acc[j];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Real <-> complex coefficient bases (per-l unitary T, at most 2 nonzeros per row)
+// ---------------------------------------------------------------------------------------------
+// SO(3) forward epilogue: Fr(l,m,n) = Re sum_{a=+-m, b=+-n} T(m,a) Fc(a,b) conj T(n,b)
+//   (src/fdcl_FFTSO3.cpp:1169-1181).   Inverse prologue: Fc(a,b) = sum_{m=+-a,n=+-b} conj T(m,a) Fr(m,n) T(n,b).
+template <typename T, int DIR>
+__global__ void so3_tmix(ChunkArgs ca, const void *src, void *dst, long ws_stride)
+{
+    typedef typename cplx_t<T>::type T2;
+    const int B = ca.B;
+    const long nc = so3_ncoef(B);
+    const long total = nc * ca.nfc;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int fi = (int)(e / nc);
+        int idx = (int)(e - (long)fi * nc);
+        int l = (int)cbrtf((float)idx * 0.75f);
+        while (so3_off(l + 1) <= idx) l++;
+        while (so3_off(l) > idx) l--;
+        int r = idx - so3_off(l), w = 2 * l + 1;
+        int m = r % w - l, n = r / w - l;
+        const long fo = ca.coef_off[ca.fn[fi]];
+        if (DIR == 0) {  // complex workspace -> real output
+            const T2 *Fc = reinterpret_cast<const T2 *>(src) + (long)fi * ws_stride;
+            T acc = (T)0;
+            for (int ia = 0; ia < (m ? 2 : 1); ia++) {
+                int av = ia ? -m : m;
+                T2 tm = tmat<T2>(m, av);
+                for (int ib = 0; ib < (n ? 2 : 1); ib++) {
+                    int bv = ib ? -n : n;
+                    T2 tn = tmat<T2>(n, bv);
+                    T2 f = Fc[so3_idx(l, av, bv)];
+                    T2 x = cmul(cmul(tm, f), cmk<T2>(tn.x, -tn.y));
+                    acc += x.x;
+                }
+            }
+            reinterpret_cast<T *>(dst)[fo + idx] = acc;
+        } else {         // real input -> complex workspace
+            const T *Fr = reinterpret_cast<const T *>(src) + fo;
+            T2 acc = cmk<T2>((T)0, (T)0);
+            // (a,b) = (m,n) here: sum over rows mm in {a,-a}, cols nn in {b,-b}
+            for (int ia = 0; ia < (m ? 2 : 1); ia++) {
+                int mm = ia ? -m : m;
+                T2 tm = tmat<T2>(mm, m);
+                for (int ib = 0; ib < (n ? 2 : 1); ib++) {
+                    int nn = ib ? -n : n;
+                    T2 tn = tmat<T2>(nn, n);
+                    T f = Fr[so3_idx(l, mm, nn)];
+                    T2 x = cmul(cmk<T2>(tm.x * f, -tm.y * f), tn);
+                    acc = cadd(acc, x);
+                }
+            }
+            reinterpret_cast<T2 *>(dst)[(long)fi * ws_stride + idx] = acc;
+        }
+    }
+}
+
+// S^2: forward Fr(l,a) = Re sum_{c=+-a} conj T(a,c) Fc(l,c) (src/fdcl_FFTS2.cpp:424-436);
+// inverse Fc(l,a) = sum_{c=+-a} T(c,a) Fr(l,c).
+template <typename T, int DIR>
+__global__ void s2_tmix(ChunkArgs ca, const void *src, void *dst, long ws_stride)
+{
+    typedef typename cplx_t<T>::type T2;
+    const int B = ca.B;
+    const long nc = (long)B * B;
+    const long total = nc * ca.nfc;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        int fi = (int)(e / nc);
+        int idx = (int)(e - (long)fi * nc);
+        int l = (int)sqrtf((float)idx);
+        while ((l + 1) * (l + 1) <= idx) l++;
+        while (l * l > idx) l--;
+        int a = idx - l * l - l;
+        const long fo = ca.coef_off[ca.fn[fi]];
+        if (DIR == 0) {
+            const T2 *Fc = reinterpret_cast<const T2 *>(src) + (long)fi * ws_stride;
+            T acc = (T)0;
+            for (int ic = 0; ic < (a ? 2 : 1); ic++) {
+                int c = ic ? -a : a;
+                T2 t = tmat<T2>(a, c);
+                T2 f = Fc[s2_idx(l, c)];
+                acc += t.x * f.x + t.y * f.y;  // Re(conj(t) f)
+            }
+            reinterpret_cast<T *>(dst)[fo + idx] = acc;
+        } else {
+            const T *Fr = reinterpret_cast<const T *>(src) + fo;
+            T2 acc = cmk<T2>((T)0, (T)0);
+            for (int ic = 0; ic < (a ? 2 : 1); ic++) {
+                int c = ic ? -a : a;
+                T2 t = tmat<T2>(c, a);
+                T f = Fr[s2_idx(l, c)];
+                acc = cadd(acc, cmk<T2>(t.x * f, t.y * f));
+            }
+            reinterpret_cast<T2 *>(dst)[(long)fi * ws_stride + idx] = acc;
+        }
+    }
+}

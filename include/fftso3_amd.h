@@ -1,0 +1,118 @@
+/*
+ * fftso3_amd.h -- C ABI of the MI355X (gfx950) SO(3) / S^2 harmonic-transform library.
+ *
+ * The reference (fdcl-gwu/FFTSO3) is a C++ class library with no C ABI; the entry points below
+ * are what its class methods bind to (the C++ class layer in fftso3_amd/cpp/ calls them):
+ *
+ *   fftso3_plan_create        <- fdcl::FFTSO3_complex(int l_max) / init(l_max)
+ *                                (include/fdcl_FFTSO3.hpp:45-53, src/fdcl_FFTSO3.cpp:3-14);
+ *                                fdcl::FFTSO3_real (include/fdcl_FFTSO3.hpp:193-202);
+ *                                fdcl::FFTS2_complex / FFTS2_real (include/fdcl_FFTS2.hpp:37-45,117-124)
+ *   fftso3_forward            <- forward_transform(std::function) (include/fdcl_FFTSO3.hpp:76,84,232,240;
+ *                                include/fdcl_FFTS2.hpp:59,136; src/fdcl_FFTSO3.cpp:654-850;
+ *                                src/fdcl_FFTS2.cpp:125-192,424-436), taking the sampled grid
+ *   fftso3_inverse            <- inverse_transform(F, point) (include/fdcl_FFTSO3.hpp:90,96,246,252;
+ *                                include/fdcl_FFTS2.hpp:65,142; src/fdcl_FFTSO3.cpp:498-531,1141-1157;
+ *                                src/fdcl_FFTS2.cpp:194-230,500-512), evaluated on the whole grid
+ *   fftso3_plan_destroy       <- ~FFTSO3_complex() (include/fdcl_FFTSO3.hpp:50)
+ *
+ * Conventions (identical to the reference):
+ *   B = l_max + 1, N = 2B; alpha_j = gamma_j = j pi / B, beta_k = (2k+1) pi / (4B).
+ *   SO(3) coefficients: F(l,m,n) at off(l) + (m+l) + (n+l)(2l+1), off(l) = l(4l^2-1)/3
+ *     (the column-major per-l blocks of FFTSO3_matrix, src/fdcl_FFTSO3_matrix.cpp:43-47);
+ *     B(4B^2-1)/3 coefficients per function.
+ *   S^2 coefficients: F(l,m) at l^2 + m + l (src/fdcl_FFTS2_matrix.cpp:43-47); B^2 per function.
+ *   SO(3) grid: [k][j1][j2] = f(alpha_j1, beta_k, gamma_j2), N^3 values per function.
+ *   S^2 grid:   [k][j]      = f(theta_k, phi_j),            N^2 values per function.
+ *   Complex values are interleaved (re, im).  Functions of a batch are packed back to back in
+ *   batch order (uneven plans: each with its own N), both for grids and for coefficients.
+ *   Element type: FFTSO3_FP64 plans use double / complex double; every other precision uses
+ *   float / complex float in memory.
+ *
+ * Ownership and threading: the caller owns the device grids and coefficient arrays; the plan owns
+ * its tables and workspaces.  A plan is stream-ordered and may be used by one host thread at a
+ * time; distinct plans are independent.  Entry points never abort: they return a status code and
+ * fftso3_last_error() describes the last failure of the calling thread.
+ */
+#ifndef FFTSO3_AMD_H
+#define FFTSO3_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fftso3_plan_s *fftso3_plan_t;
+
+enum fftso3_status {
+    FFTSO3_OK = 0,
+    FFTSO3_EINVAL = 1,        /* bad argument (bandlimit, count, null pointer, kind ...) */
+    FFTSO3_ENOMEM = 2,        /* device or host allocation failed */
+    FFTSO3_EHIP = 3,          /* a HIP runtime call or kernel launch failed */
+    FFTSO3_EUNSUPPORTED = 4   /* valid request this build does not implement */
+};
+
+enum fftso3_domain { FFTSO3_DOMAIN_SO3 = 0, FFTSO3_DOMAIN_S2 = 1 };
+enum fftso3_kind { FFTSO3_COMPLEX = 0, FFTSO3_REAL = 1 };
+
+/* Arithmetic of the beta contraction (the alpha/gamma FFTs run in the storage type; the
+ * Wigner-d / Legendre recursion, quadrature weights and normalisations are always FP64). */
+enum fftso3_precision {
+    FFTSO3_FP64 = 0,     /* double storage, FP64 everywhere: the exact drop-in */
+    FFTSO3_FP32 = 1,     /* float storage, FP32 contraction */
+    FFTSO3_BF16X3 = 2,   /* float storage, contraction as split-BF16 (2 pieces, 3 products) on MFMA */
+    FFTSO3_BF16X6 = 3    /* float storage, contraction as split-BF16 (3 pieces, 6 products) on MFMA */
+};
+
+/* --- plans ----------------------------------------------------------------------------------- */
+
+/* Uniform batch: n_fn functions of bandlimit B = l_max + 1.  flags: reserved, pass 0. */
+int fftso3_plan_create(fftso3_plan_t *plan, int domain, int l_max, int n_fn, int kind, int precision,
+                       unsigned flags);
+
+/* Uneven batch: function i has bandlimit l_max[i] + 1. */
+int fftso3_plan_create_uneven(fftso3_plan_t *plan, int domain, const int *l_max, int n_fn, int kind,
+                              int precision, unsigned flags);
+
+int fftso3_plan_destroy(fftso3_plan_t plan);
+
+/* Number of functions a plan processes per pass through the pipeline (tuning knob; 1, 2, 4 or 8). */
+int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk);
+int fftso3_plan_get_chunk(fftso3_plan_t plan, int *chunk);
+
+/* Element offsets of function i in the packed grid / coefficient arrays, and the totals. */
+long fftso3_plan_grid_offset(fftso3_plan_t plan, int fn);
+long fftso3_plan_coef_offset(fftso3_plan_t plan, int fn);
+long fftso3_plan_grid_elems(fftso3_plan_t plan);
+long fftso3_plan_coef_elems(fftso3_plan_t plan);
+/* Device workspace held by the plan, in bytes. */
+long fftso3_plan_workspace_bytes(fftso3_plan_t plan);
+
+/* --- transforms (device pointers; stream is a hipStream_t, NULL for the default stream) ------- */
+
+/* grid -> coefficients (forward_transform) */
+int fftso3_forward(fftso3_plan_t plan, const void *d_grid, void *d_coef, void *stream);
+/* coefficients -> grid (inverse_transform at every grid point) */
+int fftso3_inverse(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream);
+
+/* --- diagnostics ----------------------------------------------------------------------------- */
+
+const char *fftso3_status_string(int status);
+const char *fftso3_last_error(void);
+
+/* Per-stage kernel timing with HIP events on the launch stream.  Stages:
+ * 0 fft(forward) 1 contraction(forward) 2 basis(forward) 3 basis(inverse) 4 contraction(inverse)
+ * 5 fft(inverse).  read() synchronises, adds the elapsed milliseconds and launch counts since the
+ * last reset into ms[6] / count[6], and resets. */
+int fftso3_timing_enable(fftso3_plan_t plan, int on);
+int fftso3_timing_read(fftso3_plan_t plan, double *ms, long *count);
+
+/* Sizes (no plan needed) */
+long fftso3_so3_ncoef(int l_max);   /* (l_max+1)(4(l_max+1)^2-1)/3 */
+long fftso3_s2_ncoef(int l_max);    /* (l_max+1)^2 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

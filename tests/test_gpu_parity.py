@@ -1,0 +1,167 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle on the same seeded inputs.
+
+Tolerances: the FP64 path must match the oracle to 1e-10 relative (L2 over the array); FP32
+storage with an FP32 contraction must stay within 2e-6 relative-L2 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from fftso3_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 2e-6
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _to_dev(torch, arr, dtype):
+    return torch.from_numpy(np.ascontiguousarray(arr).reshape(-1)).to(dtype).cuda()
+
+
+def so3_roundtrip(fftso3, torch, B, F, kind="complex", prec="fp64", grid_in=None):
+    """F: [n_fn, ncoef] numpy.  Returns (GPU inverse grid, GPU forward of grid_in or of the GPU grid)."""
+    n_fn = F.shape[0]
+    plan = fftso3.Plan(B - 1, n_fn, "so3", kind, prec)
+    gdt, cdt = plan.dtypes()
+    g = plan.inverse(_to_dev(torch, F, cdt))
+    src = g if grid_in is None else _to_dev(torch, grid_in, gdt)
+    c = plan.forward(src)
+    torch.cuda.synchronize()
+    N = 2 * B
+    return g.cpu().numpy().reshape(n_fn, N, N, N), c.cpu().numpy().reshape(n_fn, -1)
+
+
+def test_so3_complex_fp64_b16(fftso3, torch_cuda, oracle):
+    """Complex SO(3), l_max=15, seed 0: inverse to the 32^3 grid, forward back, FP64."""
+    B = 16
+    F = inputs.workload_complex_b16()
+    g_ref = oracle.so3_inverse(B, F[0])
+    F_ref = oracle.so3_forward(B, g_ref)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, F, grid_in=g_ref[None])
+    assert rel_l2(g[0], g_ref) < FP64_TOL
+    assert rel_l2(c[0], F_ref) < FP64_TOL
+    assert rel_l2(c[0], F[0]) < FP64_TOL
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 6, 11, 21, 24, 32])
+def test_so3_complex_fp64_sizes(fftso3, torch_cuda, oracle, B):
+    """Bandlimits whose 2B is 2, 4, 6, 12, 22 (direct DFT), 42, 48 and 64."""
+    F = inputs.so3_coefficients(B, 1, 100 + B)
+    g_ref = oracle.so3_inverse(B, F[0])
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, F, grid_in=g_ref[None])
+    assert rel_l2(g[0], g_ref) < FP64_TOL
+    assert rel_l2(c[0], oracle.so3_forward(B, g_ref)) < FP64_TOL
+
+
+def test_so3_real_fp64_b64(fftso3, torch_cuda, oracle):
+    """Real SO(3), l_max=63, seed 1: 128^3 grid, FP64 GPU vs oracle at 1e-10."""
+    B = 64
+    Fr = inputs.workload_real_b64()
+    g_ref = oracle.so3_real_inverse(B, Fr[0])
+    F_ref = oracle.so3_real_forward(B, g_ref)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, Fr, kind="real", grid_in=g_ref[None])
+    assert rel_l2(g[0], g_ref) < FP64_TOL
+    assert rel_l2(c[0], F_ref) < FP64_TOL
+    assert rel_l2(c[0], Fr[0]) < FP64_TOL
+
+
+@pytest.mark.parametrize("B", [2, 5, 12])
+def test_so3_real_fp64_small(fftso3, torch_cuda, oracle, B):
+    Fr = inputs.so3_coefficients(B, 2, 200 + B, complex_=False)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, Fr, kind="real")
+    for i in range(2):
+        assert rel_l2(g[i], oracle.so3_real_inverse(B, Fr[i])) < FP64_TOL
+        assert rel_l2(c[i], Fr[i]) < FP64_TOL
+
+
+def test_so3_fp32_b64_batch(fftso3, torch_cuda, oracle):
+    """FP32 storage and contraction at l_max=63 on the first 3 functions of the seed-2 batch."""
+    B = 64
+    F = inputs.workload_batch_b64(3)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, F, prec="fp32")
+    for i in range(3):
+        g_ref = oracle.so3_inverse(B, F[i])
+        assert rel_l2(g[i], g_ref) < FP32_TOL
+        assert rel_l2(c[i], F[i]) < FP32_TOL
+
+
+def test_so3_chunk_tail(fftso3, torch_cuda, oracle):
+    """5 functions through chunks of 4 (a full chunk and a tail of 1)."""
+    B = 6
+    F = inputs.so3_coefficients(B, 5, 7)
+    plan = fftso3.Plan(B - 1, 5, "so3", "complex", "fp64", chunk=4)
+    torch = torch_cuda
+    g = plan.inverse(_to_dev(torch, F, torch.complex128))
+    c = plan.forward(g)
+    g = g.cpu().numpy().reshape(5, -1)
+    c = c.cpu().numpy().reshape(5, -1)
+    for i in range(5):
+        assert rel_l2(g[i], oracle.so3_inverse(B, F[i]).reshape(-1)) < FP64_TOL
+        assert rel_l2(c[i], F[i]) < FP64_TOL
+
+
+def test_so3_uneven_batch(fftso3, torch_cuda, oracle):
+    """Uneven bandlimits in one plan; functions packed back to back."""
+    torch = torch_cuda
+    Bs = [3, 8, 5, 8, 16, 2]
+    Fs = [inputs.so3_coefficients(B, 1, 300 + i)[0] for i, B in enumerate(Bs)]
+    plan = fftso3.Plan([B - 1 for B in Bs], domain="so3", kind="complex", precision="fp64")
+    flat = np.concatenate(Fs)
+    g = plan.inverse(_to_dev(torch, flat, torch.complex128))
+    c = plan.forward(g)
+    g = g.cpu().numpy()
+    c = c.cpu().numpy()
+    for i, B in enumerate(Bs):
+        go, co = plan.grid_offset(i), plan.coef_offset(i)
+        gi = g[go:go + (2 * B) ** 3]
+        ci = c[co:co + len(Fs[i])]
+        assert rel_l2(gi, oracle.so3_inverse(B, Fs[i]).reshape(-1)) < FP64_TOL
+        assert rel_l2(ci, Fs[i]) < FP64_TOL
+
+
+def s2_roundtrip(fftso3, torch, B, F, kind="complex", prec="fp64", grid_in=None):
+    n_fn = F.shape[0]
+    plan = fftso3.Plan(B - 1, n_fn, "s2", kind, prec)
+    gdt, cdt = plan.dtypes()
+    g = plan.inverse(_to_dev(torch, F, cdt))
+    src = g if grid_in is None else _to_dev(torch, grid_in, gdt)
+    c = plan.forward(src)
+    torch.cuda.synchronize()
+    N = 2 * B
+    return g.cpu().numpy().reshape(n_fn, N, N), c.cpu().numpy().reshape(n_fn, -1)
+
+
+@pytest.mark.parametrize("B", [1, 3, 6, 16, 21, 64])
+def test_s2_complex_fp64(fftso3, torch_cuda, oracle, B):
+    F = inputs.s2_coefficients(B, 2, 400 + B)
+    g_ref = np.stack([oracle.s2_inverse(B, F[i]) for i in range(2)])
+    g, c = s2_roundtrip(fftso3, torch_cuda, B, F, grid_in=g_ref)
+    for i in range(2):
+        assert rel_l2(g[i], g_ref[i]) < FP64_TOL
+        assert rel_l2(c[i], oracle.s2_forward(B, g_ref[i])) < FP64_TOL
+        assert rel_l2(c[i], F[i]) < FP64_TOL
+
+
+@pytest.mark.parametrize("B", [2, 6, 32])
+def test_s2_real_fp64(fftso3, torch_cuda, oracle, B):
+    F = inputs.s2_coefficients(B, 2, 500 + B, complex_=False)
+    g, c = s2_roundtrip(fftso3, torch_cuda, B, F, kind="real")
+    for i in range(2):
+        assert rel_l2(g[i], oracle.s2_real_inverse(B, F[i])) < FP64_TOL
+        assert rel_l2(c[i], oracle.s2_real_forward(B, oracle.s2_real_inverse(B, F[i]))) < FP64_TOL
+
+
+def test_s2_fp32_b128(fftso3, torch_cuda, oracle):
+    """S^2 at l_max=127 (256x256 grid), first 4 fields of the seed-5 workload, FP32."""
+    B = 128
+    F = inputs.workload_s2(4)
+    g, c = s2_roundtrip(fftso3, torch_cuda, B, F, prec="fp32")
+    for i in range(4):
+        assert rel_l2(g[i], oracle.s2_inverse(B, F[i])) < FP32_TOL
+        assert rel_l2(c[i], F[i]) < FP32_TOL
