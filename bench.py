@@ -1,0 +1,186 @@
+"""Benchmark: SO(3) forward+inverse transforms/s at l_max=63 (B=64), batch 256, one MI355X.
+
+A step is one inverse transform of the whole batch (coefficients -> 256 x 128^3 grids) followed
+by one forward transform (grids -> coefficients), with inputs resident in HBM.  Prints one JSON
+line.  Also measured: the FP64 GPU path on the same batch, the error of the fast path against
+the FP64 GPU path (all 256 functions) and against the CPU oracle (function 0), per-kernel HIP
+event timings on the launch stream, and the CPU oracle's own throughput on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SO(3) fwd+inv transforms/s, B=64 batch 256: split-FP32 vs FP64 @≤1e-6 rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def alg_bytes(B, n_fn, esz, stage):
+    """Algorithmic HBM bytes of one launch of a stage over n_fn functions (complex elements of
+    esz bytes): grid N^3, unique spectrum N(2B-1)^2, coefficients B(4B^2-1)/3."""
+    N = 2 * B
+    grid = N ** 3
+    spec = N * (2 * B - 1) ** 2
+    coef = B * (4 * B * B - 1) // 3
+    per = {"fft_fwd": grid + spec, "contract_fwd": spec + coef,
+           "contract_inv": coef + spec, "fft_inv": spec + grid}[stage]
+    return n_fn * per * esz
+
+
+def cpu_baseline(budget_s=12.0):
+    from oracle import oracle
+    from fftso3_amd import inputs
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    B = 64
+    F = inputs.workload_batch_b64(4)
+    t0 = time.perf_counter()
+    done = 0
+    while done < 4:
+        g = oracle.so3_inverse(B, F[done], cores)
+        oracle.so3_forward(B, g, cores)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "transforms/s", "cores": cores, "kind": "port",
+            "sample": "%d of the 256 functions (l_max=63, complex), FP64 oracle grid inverse + forward "
+                      "(OpenMP over beta, %d threads), %.1f s" % (done, cores, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--chunk", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fftso3_amd
+    from fftso3_amd import inputs
+
+    dev = torch.device("cuda:0")
+    B, n_fn = 64, args.batch
+    log("generating %d seeded coefficient sets (l_max=63)" % n_fn)
+    F64 = inputs.workload_batch_b64(n_fn)
+    Fd = torch.from_numpy(F64.reshape(-1)).to(dev)            # complex128
+    Fs = Fd.to(torch.complex64)
+
+    plan = fftso3_amd.Plan(B - 1, n_fn, "so3", "complex", args.precision, chunk=args.chunk)
+    grid = torch.empty(plan.grid_elems(), dtype=torch.complex64, device=dev)
+    Fout = torch.empty_like(Fs)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        plan.inverse(Fs, grid, stream)
+        plan.forward(grid, Fout, stream)
+
+    log("warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    plan.timing(True)
+    log("timing %d steps" % args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = plan.timing_read()
+    plan.timing(False)
+    value = n_fn * args.steps / dt
+    log("fast path: %.1f transforms/s (%.3f ms/step)" % (value, 1e3 * dt / args.steps))
+
+    # error of the fast path
+    rt_err = float(torch.linalg.norm((Fout.to(torch.complex128) - Fd)) / torch.linalg.norm(Fd))
+    extra = {"precision": args.precision, "chunk": plan.chunk(),
+             "roundtrip_rel_l2": rt_err,
+             "stage_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
+             "stage_launches_per_step": {k: v[1] / args.steps for k, v in stages.items()}}
+    if not args.no_fp64:
+        log("FP64 GPU path")
+        p64 = fftso3_amd.Plan(B - 1, n_fn, "so3", "complex", "fp64")
+        g64 = torch.empty(p64.grid_elems(), dtype=torch.complex128, device=dev)
+        F64o = torch.empty_like(Fd)
+        p64.inverse(Fd, g64)
+        p64.forward(g64, F64o)
+        torch.cuda.synchronize()
+        k64 = max(1, min(3, args.steps))
+        t0 = time.perf_counter()
+        for _ in range(k64):
+            p64.inverse(Fd, g64)
+            p64.forward(g64, F64o)
+        torch.cuda.synchronize()
+        v64 = n_fn * k64 / (time.perf_counter() - t0)
+        plan.inverse(Fs, grid)
+        plan.forward(g64.to(torch.complex64), Fout)
+        torch.cuda.synchronize()
+        gerr = float(torch.linalg.norm(grid.to(torch.complex128) - g64) / torch.linalg.norm(g64))
+        ferr = float(torch.linalg.norm(Fout.to(torch.complex128) - F64o) / torch.linalg.norm(F64o))
+        ferr_max = float((Fout.to(torch.complex128) - F64o).abs().max())
+        extra.update({"fp64_gpu_value": v64, "speedup_vs_fp64_gpu": value / v64,
+                      "inverse_grid_rel_l2_vs_fp64": gerr, "forward_coef_rel_l2_vs_fp64": ferr,
+                      "forward_coef_max_abs_vs_fp64": ferr_max})
+        del g64, p64
+        log("FP64 GPU path: %.1f transforms/s; errors grid %.2e coef %.2e rt %.2e" % (v64, gerr, ferr, rt_err))
+        # function 0 against the CPU oracle
+        from oracle import oracle
+        g_ref = oracle.so3_inverse(B, F64[0])
+        g0 = grid[: (2 * B) ** 3].cpu().numpy().astype(np.complex128)
+        extra["oracle_fn0_grid_rel_l2"] = float(np.linalg.norm(g0 - g_ref.reshape(-1)) / np.linalg.norm(g_ref))
+
+    # roofline of the dominant kernel (stage with the largest time)
+    dom = max(("fft_fwd", "contract_fwd", "contract_inv", "fft_inv"), key=lambda s: stages[s][0])
+    ms, cnt = stages[dom]
+    avg_ms = ms / cnt
+    C = plan.chunk()
+    bytes_launch = alg_bytes(B, C, 8, dom)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "alg_bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms}
+    whole = 39146496 * value / 1e9
+    extra["pipeline_alg_GBs"] = whole
+    extra["pipeline_roofline_frac"] = whole / HBM_PEAK_GBS
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU oracle baseline")
+        cpu = cpu_baseline()
+        extra["speedup_vs_cpu"] = value / cpu["value"]
+
+    out = {"metric": METRIC, "value": value, "unit": "transforms/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+           "config": {"workload": "batched complex SO(3) transform, l_max=63 (B=64), 256 functions, "
+                                  "complex64 grids 256x128^3, coefficients N(0,1/(1+l)^2) seed 2",
+                      "l_max": 63, "batch": n_fn, "grid": "256x128^3 complex64", "precision": args.precision},
+           "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
