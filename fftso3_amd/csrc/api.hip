@@ -123,6 +123,7 @@ FftDesc make_fft_desc(int N, std::vector<int> &rev)
 {
     FftDesc fd{};
     fd.N = N;
+    fd.lgN = (N & (N - 1)) == 0 ? __builtin_ctz(N) : -1;
     int n = N, e = 0;
     while (n % 2 == 0) { n /= 2; e++; }
     std::vector<int> rad;
@@ -147,6 +148,8 @@ FftDesc make_fft_desc(int N, std::vector<int> &rev)
         fd.R[t] = rad[t];
         fd.M[t] = L / rad[t];
         L = fd.M[t];
+        fd.lgR[t] = (fd.R[t] & (fd.R[t] - 1)) == 0 ? __builtin_ctz(fd.R[t]) : -1;
+        fd.lgM[t] = (fd.M[t] & (fd.M[t] - 1)) == 0 ? __builtin_ctz(fd.M[t]) : -1;
     }
     for (int f = 0; f < N; f++) {
         int rem = f, pos = 0;
@@ -261,9 +264,10 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     for (int i = 0; i < C; i++) wsoff[i] = (long)i * nc;
     if ((st = c->d_ident.upload(ident)) || (st = c->d_ws_off.upload(wsoff))) return st;
     // FFT mode
-    size_t slice = (size_t)N * N * pl->esz;
+    const size_t tables = (size_t)N * (pl->esz + sizeof(int));
+    size_t slice = (size_t)N * N * pl->esz + tables;
     c->fused = !s2 && slice <= (size_t)kMaxLds && (size_t)N * N <= (size_t)FFT_EPT * kFftThreads;
-    const size_t cap = std::min((size_t)kMaxLds / pl->esz, (size_t)FFT_EPT * kFftThreads);
+    const size_t cap = std::min(((size_t)kMaxLds - tables) / pl->esz, (size_t)FFT_EPT * kFftThreads);
     int rt = 1;
     while (rt * 2 <= N && N % (rt * 2) == 0 && (size_t)(rt * 2) * N <= cap) rt *= 2;
     c->row_tile = rt;
@@ -335,6 +339,24 @@ struct StageTimer {
         if (idx >= 0) (void)hipEventRecord(pl->ev_pool[idx].second, s);
     }
 };
+
+int g_num_cus = 0;
+
+// persistent grid: every CU filled to its occupancy for this kernel and LDS size, multiple of 8
+template <typename K>
+int persistent_grid(K kernel, size_t lds, int nunits)
+{
+    if (g_num_cus == 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) g_num_cus = 256;
+    }
+    int per = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kFftThreads, lds) != hipSuccess || per < 1) per = 1;
+    int grid = g_num_cus * per;
+    grid = std::min(grid, xcd_grid(nunits));
+    return std::max(XCD_COUNT, (grid / XCD_COUNT) * XCD_COUNT);
+}
 
 template <typename T>
 ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
@@ -435,6 +457,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
     const long nc = s2 ? (long)c->B * c->B : so3_ncoef(c->B);
     int st;
     const int tmix_blocks = (int)std::min<long>(4096, (nc * nfc + 255) / 256);
+    const size_t tables = (size_t)N * (sizeof(T2) + sizeof(int));
     if (fwd) {
         {
             StageTimer t(pl, s, 0);
@@ -442,23 +465,28 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             if (s2) {
                 a.tile = c->row_tile;
                 int nunit = nfc * (N / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((s2_fft_fwd<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((s2_fft_fwd<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
             } else if (c->fused) {
-                size_t lds = (size_t)N * N * sizeof(T2);
-                if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                size_t lds = (size_t)N * N * sizeof(T2) + tables;
+                if (real) {
+                    auto k = so3_fft_fwd_slice<T, true>;
+                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
+                } else {
+                    auto k = so3_fft_fwd_slice<T, false>;
+                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
+                }
             } else {
                 a.tile = c->row_tile;
                 int nunit = nfc * N * (N / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((so3_fft_rows<T, false, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 HIP_TRY(hipGetLastError());
                 a.tile = c->col_tile;
                 nunit = nfc * N * (N / a.tile);
-                lds = (size_t)a.tile * N * sizeof(T2);
+                lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 hipLaunchKernelGGL((so3_fft_cols<T, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
             }
             HIP_TRY(hipGetLastError());
@@ -492,22 +520,27 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             if (s2) {
                 a.tile = c->row_tile;
                 int nunit = nfc * (N / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((s2_fft_inv<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((s2_fft_inv<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
             } else if (c->fused) {
-                size_t lds = (size_t)N * N * sizeof(T2);
-                if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                size_t lds = (size_t)N * N * sizeof(T2) + tables;
+                if (real) {
+                    auto k = so3_fft_inv_slice<T, true>;
+                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
+                } else {
+                    auto k = so3_fft_inv_slice<T, false>;
+                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
+                }
             } else {
                 a.tile = c->col_tile;
                 int nunit = nfc * N * (N / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2);
+                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 hipLaunchKernelGGL((so3_fft_cols<T, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 HIP_TRY(hipGetLastError());
                 a.tile = c->row_tile;
                 nunit = nfc * N * (N / a.tile);
-                lds = (size_t)a.tile * N * sizeof(T2);
+                lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((so3_fft_rows<T, false, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
             }

@@ -19,10 +19,15 @@
 
 struct FftDesc {
     int N;
+    int lgN;                   // log2 N when N is a power of two, else -1
     int nst;                   // number of DIF stages (0 => direct DFT)
     int R[FFT_MAX_STAGES];     // radix of stage t
     int M[FFT_MAX_STAGES];     // M_t = L_t / R_t, L_t = span of stage t (L_0 = N)
+    int lgR[FFT_MAX_STAGES];   // log2 R_t (power-of-two N only)
+    int lgM[FFT_MAX_STAGES];   // log2 M_t (power-of-two N only)
 };
+
+__device__ __forceinline__ int ilog2_or_neg(int x) { return (x > 0 && (x & (x - 1)) == 0) ? __ffs(x) - 1 : -1; }
 
 // small in-register DFTs: v[s] <- sum_r v[r] exp(-2 pi i r s / R)
 template <typename T2> __device__ __forceinline__ void dft2(T2 *v)
@@ -103,11 +108,22 @@ __device__ __forceinline__ void dif_stage(T2 *buf, const FftDesc &fd, int t, int
     const int per = N / R;           // butterflies per transform
     const int total = per * nb;
     const int twstep = N / Lt;
+    const int lgnb = ilog2_or_neg(nb);
+    const bool p2 = fd.lgN >= 0 && lgnb >= 0;
+    const int lgper = fd.lgN - fd.lgR[t], lgM = fd.lgM[t];
     for (int beta = tid; beta < total; beta += nthr) {
-        int b, w;
-        if (jfast) { b = beta / per; w = beta - b * per; }
-        else { w = beta / nb; b = beta - w * nb; }
-        int blk = w / M, j = w - blk * M;
+        int b, w, blk, j;
+        if (p2) {
+            if (jfast) { b = beta >> lgper; w = beta & (per - 1); }
+            else { w = beta >> lgnb; b = beta & (nb - 1); }
+            blk = w >> lgM;
+            j = w & (M - 1);
+        } else {
+            if (jfast) { b = beta / per; w = beta - b * per; }
+            else { w = beta / nb; b = beta - w * nb; }
+            blk = w / M;
+            j = w - blk * M;
+        }
         int base = b * bs + (blk * Lt + j) * es;
         T2 v[R];
 #pragma unroll
@@ -116,7 +132,8 @@ __device__ __forceinline__ void dif_stage(T2 *buf, const FftDesc &fd, int t, int
         // twiddle W_Lt^(s j) = W_N^(s j N/Lt)
 #pragma unroll
         for (int s = 1; s < R; s++) {
-            int e = (s * j * twstep) % N;
+            int e = s * j * twstep;
+            e = p2 ? (e & (N - 1)) : (e % N);
             v[s] = cmul(v[s], tw[e]);
         }
 #pragma unroll

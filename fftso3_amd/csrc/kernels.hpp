@@ -51,77 +51,152 @@ template <typename T> struct FftArgs {
     int tile;                            // two-pass fallback: rows (or columns) per workgroup
 };
 
-// Fused forward: one (fi, k) slice per workgroup, whole N x N slice in LDS.
+// Copy n elements into LDS with U loads in flight per thread before the first store.
+template <int U, typename T2, typename F>
+__device__ __forceinline__ void fill_lds(T2 *dst, int n, int tid, int nthr, F get)
+{
+    for (int base = 0; base < n; base += U * nthr) {
+        T2 r[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int e = base + tid + u * nthr;
+            if (e < n) r[u] = get(e);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            int e = base + tid + u * nthr;
+            if (e < n) dst[e] = r[u];
+        }
+    }
+}
+
+// twiddle and digit-reversal tables follow the data in LDS
+template <typename T2>
+__device__ __forceinline__ void load_fft_tables(T2 *twl, int *revl, const T2 *tw, const int *rev, int N, int tid, int nthr)
+{
+    for (int i = tid; i < N; i += nthr) {
+        twl[i] = tw[i];
+        revl[i] = rev[i];
+    }
+}
+
+// Persistent workgroups walk the (k, fi) slices of the chunk; the units that are in flight at
+// the same time on one XCD are consecutive (k, fi) pairs, so their workspace writes share L2
+// lines.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ int persistent_first_unit()
+{
+    const int perx = gridDim.x / XCD_COUNT;
+    return (blockIdx.x % XCD_COUNT) * perx + blockIdx.x / XCD_COUNT;
+}
+
+// Fused forward: one (fi, k) slice at a time, whole N x N slice in LDS; the next slice is
+// prefetched into registers while the current one is transformed.
 template <typename T, bool REAL>
 __global__ void __launch_bounds__(1024) so3_fft_fwd_slice(FftArgs<T> a)
 {
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     const ChunkArgs &ca = a.ca;
-    const int N = ca.N, B = ca.B, nslice = ca.nfc * N;
-    const int unit = xcd_unit(nslice);
-    if (unit >= nslice) return;
-    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
-    const int fn = ca.fn[fi];
+    const int N = ca.N, B = ca.B, NN = N * N, nslice = ca.nfc * N;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    T2 *twl = buf + NN;
+    int *revl = reinterpret_cast<int *>(twl + N);
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const long base = ca.grid_off[fn] + (long)k * N * N;
-    if (REAL) {
-        const T *g = reinterpret_cast<const T *>(a.src) + base;
-        for (int e = tid; e < N * N; e += nthr) buf[e] = cmk<T2>(g[e], (T)0);
-    } else {
-        const T2 *g = reinterpret_cast<const T2 *>(a.src) + base;
-        for (int e = tid; e < N * N; e += nthr) buf[e] = g[e];
-    }
-    __syncthreads();
-    fft_tile(buf, a.fd, 1, N, N, true, a.tw, tid, nthr);   // over j2 (gamma)
-    fft_tile(buf, a.fd, N, 1, N, false, a.tw, tid, nthr);  // over j1 (alpha)
-    // emit S_k(m,n) = X(-m mod N, -n mod N) to its group slot
+    load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
+    T2 r[FFT_EPT];
+    auto prefetch = [&](int unit) {
+        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+        const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
+#pragma unroll
+        for (int i = 0; i < FFT_EPT; i++) {
+            int e = tid + i * nthr;
+            if (e < NN) {
+                if (REAL) r[i] = cmk<T2>(reinterpret_cast<const T *>(a.src)[base + e], (T)0);
+                else r[i] = reinterpret_cast<const T2 *>(a.src)[base + e];
+            }
+        }
+    };
     T2 *S = reinterpret_cast<T2 *>(a.dst);
     const int nq = 4 * B * B;
-    for (int q = tid; q < nq; q += nthr) {
-        int g = q >> 2, p = q & 3, m0, n0, m, n;
-        so3_group(g, m0, n0);
-        so3_pair(m0, n0, p, m, n);
-        int u = a.rev[wrapN(-m, N)], v = a.rev[wrapN(-n, N)];
-        S[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[u * N + v];
+    int unit = persistent_first_unit();
+    if (unit < nslice) prefetch(unit);
+    for (; unit < nslice; unit += gridDim.x) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < FFT_EPT; i++) {
+            int e = tid + i * nthr;
+            if (e < NN) buf[e] = r[i];
+        }
+        __syncthreads();
+        if (unit + (int)gridDim.x < nslice) prefetch(unit + gridDim.x);
+        fft_tile(buf, a.fd, 1, N, N, true, twl, tid, nthr);   // over j2 (gamma)
+        fft_tile(buf, a.fd, N, 1, N, false, twl, tid, nthr);  // over j1 (alpha)
+        // emit S_k(m,n) = X(-m mod N, -n mod N) to its group slot
+        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+        for (int q = tid; q < nq; q += nthr) {
+            int g = q >> 2, p = q & 3, m0, n0, m, n;
+            so3_group(g, m0, n0);
+            so3_pair(m0, n0, p, m, n);
+            int u = revl[wrapN(-m, N)], v = revl[wrapN(-n, N)];
+            S[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[u * N + v];
+        }
     }
 }
 
-// Fused inverse: gather G_k(m,n) into bins (m mod N, n mod N), 2-D DFT, store the slice.
+// Fused inverse: gather G_k(m,n) into bins (m mod N, n mod N), 2-D DFT, store the slice; the
+// next slice's gather is issued before the current transform.
 template <typename T, bool REAL>
 __global__ void __launch_bounds__(1024) so3_fft_inv_slice(FftArgs<T> a)
 {
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     const ChunkArgs &ca = a.ca;
-    const int N = ca.N, B = ca.B, nslice = ca.nfc * N;
-    const int unit = xcd_unit(nslice);
-    if (unit >= nslice) return;
-    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
-    const int fn = ca.fn[fi];
+    const int N = ca.N, B = ca.B, NN = N * N, nslice = ca.nfc * N;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    T2 *twl = buf + NN;
+    int *revl = reinterpret_cast<int *>(twl + N);
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const T2 *G = reinterpret_cast<const T2 *>(a.src);
-    for (int e = tid; e < N * N; e += nthr) {
-        int u = e / N, v = e - u * N;
-        T2 val = cmk<T2>((T)0, (T)0);
-        if (u != B && v != B) {
-            int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
-            so3_slot(m, n, g, p);
-            val = G[(((long)g * N + k) * ca.C + fi) * 4 + p];
+    T2 r[FFT_EPT];
+    auto prefetch = [&](int unit) {
+        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+#pragma unroll
+        for (int i = 0; i < FFT_EPT; i++) {
+            int e = tid + i * nthr;
+            if (e < NN) {
+                int u = e / N, v = e - u * N;
+                T2 val = cmk<T2>((T)0, (T)0);
+                if (u != B && v != B) {
+                    int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
+                    so3_slot(m, n, g, p);
+                    val = G[(((long)g * N + k) * ca.C + fi) * 4 + p];
+                }
+                r[i] = val;
+            }
         }
-        buf[e] = val;
-    }
-    __syncthreads();
-    fft_tile(buf, a.fd, 1, N, N, true, a.tw, tid, nthr);
-    fft_tile(buf, a.fd, N, 1, N, false, a.tw, tid, nthr);
-    const long base = ca.grid_off[fn] + (long)k * N * N;
-    for (int e = tid; e < N * N; e += nthr) {
-        int j1 = e / N, j2 = e - j1 * N;
-        T2 val = buf[a.rev[j1] * N + a.rev[j2]];
-        if (REAL) reinterpret_cast<T *>(a.dst)[base + e] = val.x;
-        else reinterpret_cast<T2 *>(a.dst)[base + e] = val;
+    };
+    int unit = persistent_first_unit();
+    if (unit < nslice) prefetch(unit);
+    for (; unit < nslice; unit += gridDim.x) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < FFT_EPT; i++) {
+            int e = tid + i * nthr;
+            if (e < NN) buf[e] = r[i];
+        }
+        __syncthreads();
+        if (unit + (int)gridDim.x < nslice) prefetch(unit + gridDim.x);
+        fft_tile(buf, a.fd, 1, N, N, true, twl, tid, nthr);
+        fft_tile(buf, a.fd, N, 1, N, false, twl, tid, nthr);
+        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+        const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
+        for (int e = tid; e < NN; e += nthr) {
+            int j1 = e / N, j2 = e - j1 * N;
+            T2 val = buf[revl[j1] * N + revl[j2]];
+            if (REAL) reinterpret_cast<T *>(a.dst)[base + e] = val.x;
+            else reinterpret_cast<T2 *>(a.dst)[base + e] = val;
+        }
     }
 }
 
@@ -133,32 +208,36 @@ __global__ void __launch_bounds__(1024) so3_fft_rows(FftArgs<T> a)
 {
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     const ChunkArgs &ca = a.ca;
     const int N = ca.N, R = a.tile, nblk = N / R, nunit = ca.nfc * N * nblk;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    T2 *twl = buf + R * N;
+    int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
     const int rb = unit % nblk, k = (unit / nblk) % N, fi = unit / (nblk * N);
     const int fn = ca.fn[fi];
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const long gbase = ca.grid_off[fn] + ((long)k * N + (long)rb * R) * N;
     const long sbase = (((long)fi * N + k) * N + (long)rb * R) * N;
     if (MODE == 0) {
         if (REAL) {
             const T *g = reinterpret_cast<const T *>(a.src) + gbase;
-            for (int e = tid; e < R * N; e += nthr) buf[e] = cmk<T2>(g[e], (T)0);
+            fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return cmk<T2>(g[e], (T)0); });
         } else {
             const T2 *g = reinterpret_cast<const T2 *>(a.src) + gbase;
-            for (int e = tid; e < R * N; e += nthr) buf[e] = g[e];
+            fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return g[e]; });
         }
     } else {
-        for (int e = tid; e < R * N; e += nthr) buf[e] = a.scratch[sbase + e];
+        const T2 *sc = a.scratch + sbase;
+        fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return sc[e]; });
     }
     __syncthreads();
-    fft_tile(buf, a.fd, 1, N, R, true, a.tw, tid, nthr);
+    fft_tile(buf, a.fd, 1, N, R, true, twl, tid, nthr);
     for (int e = tid; e < R * N; e += nthr) {
         int r = e / N, v = e - r * N;
-        T2 val = buf[r * N + a.rev[v]];
+        T2 val = buf[r * N + revl[v]];
         if (MODE == 0) a.scratch[sbase + e] = val;
         else if (REAL) reinterpret_cast<T *>(a.dst)[gbase + e] = val.x;
         else reinterpret_cast<T2 *>(a.dst)[gbase + e] = val;
@@ -173,22 +252,26 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
 {
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);  // [row][tc]
     const ChunkArgs &ca = a.ca;
     const int N = ca.N, B = ca.B, TC = a.tile, nblk = N / TC, nunit = ca.nfc * N * nblk;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);  // [row][tc]
+    T2 *twl = buf + TC * N;
+    int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
     const int cb = unit % nblk, k = (unit / nblk) % N, fi = unit / (nblk * N);
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const long sbase = ((long)fi * N + k) * N * N;
     T2 *W = reinterpret_cast<T2 *>(MODE == 0 ? a.dst : const_cast<void *>(a.src));
     if (MODE == 0) {
-        for (int e = tid; e < N * TC; e += nthr) {
+        const T2 *sc = a.scratch + sbase;
+        fill_lds<8>(buf, N * TC, tid, nthr, [&](int e) {
             int r = e / TC, c = e - r * TC;
-            buf[e] = a.scratch[sbase + (long)r * N + cb * TC + c];
-        }
+            return sc[(long)r * N + cb * TC + c];
+        });
     } else {
-        for (int e = tid; e < N * TC; e += nthr) {
+        fill_lds<8>(buf, N * TC, tid, nthr, [&](int e) {
             int u = e / TC, c = e - u * TC, v = cb * TC + c;
             T2 val = cmk<T2>((T)0, (T)0);
             if (u != B && v != B) {
@@ -196,11 +279,11 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
                 so3_slot(m, n, g, p);
                 val = W[(((long)g * N + k) * ca.C + fi) * 4 + p];
             }
-            buf[e] = val;
-        }
+            return val;
+        });
     }
     __syncthreads();
-    fft_tile(buf, a.fd, TC, 1, TC, false, a.tw, tid, nthr);
+    fft_tile(buf, a.fd, TC, 1, TC, false, twl, tid, nthr);
     if (MODE == 0) {
         for (int e = tid; e < N * TC; e += nthr) {
             int u = e / TC, c = e - u * TC, v = cb * TC + c;
@@ -212,12 +295,12 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
             so3_slot(m, n, g, p);
             // duplicated slots of orbits of size 1 or 2 are left unwritten: the contraction's
             // columns are independent and it never stores results for invalid pairs
-            W[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[a.rev[u] * TC + c];
+            W[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[revl[u] * TC + c];
         }
     } else {
         for (int e = tid; e < N * TC; e += nthr) {
             int j1 = e / TC, c = e - j1 * TC;
-            a.scratch[sbase + (long)j1 * N + cb * TC + c] = buf[a.rev[j1] * TC + c];
+            a.scratch[sbase + (long)j1 * N + cb * TC + c] = buf[revl[j1] * TC + c];
         }
     }
 }
@@ -230,24 +313,27 @@ __global__ void __launch_bounds__(1024) s2_fft_fwd(FftArgs<T> a)
 {
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     const ChunkArgs &ca = a.ca;
     const int N = ca.N, B = ca.B, R = a.tile, nblk = N / R, nunit = ca.nfc * nblk;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    T2 *twl = buf + R * N;
+    int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
     const int rb = unit % nblk, fi = unit / nblk;
     const int fn = ca.fn[fi];
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const long gbase = ca.grid_off[fn] + (long)rb * R * N;
     if (REAL) {
         const T *g = reinterpret_cast<const T *>(a.src) + gbase;
-        for (int e = tid; e < R * N; e += nthr) buf[e] = cmk<T2>(g[e], (T)0);
+        fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return cmk<T2>(g[e], (T)0); });
     } else {
         const T2 *g = reinterpret_cast<const T2 *>(a.src) + gbase;
-        for (int e = tid; e < R * N; e += nthr) buf[e] = g[e];
+        fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return g[e]; });
     }
     __syncthreads();
-    fft_tile(buf, a.fd, 1, N, R, true, a.tw, tid, nthr);
+    fft_tile(buf, a.fd, 1, N, R, true, twl, tid, nthr);
     // S_k(m) = X(m mod N) (src/fdcl_FFTS2.cpp:159-166); slot (|m|, m<0)
     T2 *S = reinterpret_cast<T2 *>(a.dst);
     const int nq = 2 * B;
@@ -255,7 +341,7 @@ __global__ void __launch_bounds__(1024) s2_fft_fwd(FftArgs<T> a)
         int r = e / nq, q = e - r * nq, g = q >> 1, p = q & 1;
         int m = p ? -g : g;
         int k = rb * R + r;
-        S[(((long)g * N + k) * ca.C + fi) * 2 + p] = buf[r * N + a.rev[wrapN(m, N)]];
+        S[(((long)g * N + k) * ca.C + fi) * 2 + p] = buf[r * N + revl[wrapN(m, N)]];
     }
 }
 
@@ -264,33 +350,35 @@ __global__ void __launch_bounds__(1024) s2_fft_inv(FftArgs<T> a)
 {
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     const ChunkArgs &ca = a.ca;
     const int N = ca.N, B = ca.B, R = a.tile, nblk = N / R, nunit = ca.nfc * nblk;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);
+    T2 *twl = buf + R * N;
+    int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
     const int rb = unit % nblk, fi = unit / nblk;
     const int fn = ca.fn[fi];
     const int tid = threadIdx.x, nthr = blockDim.x;
+    load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const T2 *G = reinterpret_cast<const T2 *>(a.src);
     // G_k(m) at bin (-m mod N)
-    for (int e = tid; e < R * N; e += nthr) {
+    fill_lds<8>(buf, R * N, tid, nthr, [&](int e) {
         int r = e / N, u = e - r * N, k = rb * R + r;
         T2 val = cmk<T2>((T)0, (T)0);
         if (u != B) {
-            int mneg = u < B ? u : u - N;   // bin u holds m = -mneg
-            int m = -mneg;
+            int m = -(u < B ? u : u - N);   // bin u holds m with -m = u (mod N)
             int g = iabs(m), p = m < 0;
             val = G[(((long)g * N + k) * ca.C + fi) * 2 + p];
         }
-        buf[e] = val;
-    }
+        return val;
+    });
     __syncthreads();
-    fft_tile(buf, a.fd, 1, N, R, true, a.tw, tid, nthr);
+    fft_tile(buf, a.fd, 1, N, R, true, twl, tid, nthr);
     const long gbase = ca.grid_off[fn] + (long)rb * R * N;
     for (int e = tid; e < R * N; e += nthr) {
         int r = e / N, j = e - r * N;
-        T2 val = buf[r * N + a.rev[j]];
+        T2 val = buf[r * N + revl[j]];
         if (REAL) reinterpret_cast<T *>(a.dst)[gbase + e] = val.x;
         else reinterpret_cast<T2 *>(a.dst)[gbase + e] = val;
     }
@@ -390,14 +478,47 @@ template <typename T> struct ConArgs {
     int s2;               // 1 for S^2 (two pairs, Legendre recursion)
 };
 
+template <typename T> struct vec16;
+template <> struct vec16<float> { typedef float4 type; static constexpr int n = 4; };
+template <> struct vec16<double> { typedef double2 type; static constexpr int n = 2; };
+
+// Run the recursion of group g for one beta_k and hand every (l, value) with l in [m0, lend) to f.
+template <typename F>
+__device__ __forceinline__ void group_recursion(const RecTables &rt, int s2, int B, int g, int m0, int n0, int k,
+                                                int lend, F f)
+{
+    if (s2) {
+        LegendreRec r;
+        r.init(rt, B, g, k);
+        double v = r.d1;
+        for (int l = m0; l < lend; l++) {
+            if (l > m0) v = r.step(l);
+            f(l, v);
+        }
+    } else {
+        WignerRec r;
+        r.init(rt, B, g, m0, n0, k);
+        double v = r.d1;
+        for (int l = m0; l < lend; l++) {
+            if (l > m0) v = r.step(l);
+            f(l, v);
+        }
+    }
+}
+
 // Forward: F^l_{pair} = s_p sum_k w_k d^l_{m0 n0}(beta_k) S_k(pair), l in [l_start, l_start+LT).
-// Block = 32*C threads: 8C real columns (fi, p, re/im) x 4 row groups of 4 rows.  (S^2 uses
-// 4C real columns and 8 row groups.)
+// Block = 32*C threads: ncol real columns (fi, p, re/im) x row groups of 4 rows (LT = 4 * groups).
+// The spectrum tile of the group is streamed through LDS in chunks of up to 128 beta values; the
+// first chunk's loads are issued before the FP64 recursion so their latency hides behind it, and
+// each further chunk is loaded while the previous one is multiplied.
 template <typename T, int C>
 __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
 {
-    typedef typename cplx_t<T>::type T2;
+    typedef typename vec16<T>::type VT;
+    constexpr int V = vec16<T>::n;
     constexpr int NT = 32 * C;
+    constexpr int KCMAX = 128;
+    constexpr int PF = KCMAX * 8 * C / (NT * V);   // vector loads per thread per chunk (SO(3) bound)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, LT = a.tile;
@@ -410,50 +531,55 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
     int m0, n0;
     if (a.s2) { m0 = g; n0 = 0; } else so3_group(g, m0, n0);
     const int lend = min(B, l0 + LT);
+    const int KC = min(N, KCMAX);
     T *D = reinterpret_cast<T *>(smem_raw);          // [k][LT]
     T *Ss = D + (size_t)N * LT;                      // [KC][ncol]
     const int tid = threadIdx.x;
-    // phase 1: FP64 recursion, one lane per k; D[k][l-l0] = w_k d^l(beta_k)
-    for (int k = tid; k < N; k += NT) {
-        double wk = a.rt.w[k];
-        double v;
-        if (a.s2) {
-            LegendreRec r;
-            r.init(a.rt, B, g, k);
-            v = r.d1;
-            for (int l = m0; l < lend; l++) {
-                if (l > m0) v = r.step(l);
-                if (l >= l0) D[(size_t)k * LT + (l - l0)] = (T)(wk * v);
-            }
-        } else {
-            WignerRec r;
-            r.init(a.rt, B, g, m0, n0, k);
-            v = r.d1;
-            for (int l = m0; l < lend; l++) {
-                if (l > m0) v = r.step(l);
-                if (l >= l0) D[(size_t)k * LT + (l - l0)] = (T)(wk * v);
-            }
+    const VT *Sv = reinterpret_cast<const VT *>(reinterpret_cast<const T *>(a.src) + (size_t)g * N * ncol);
+    VT r[PF];
+    auto load_chunk = [&](int kc) {
+        const int nv = min(KC, N - kc) * ncol / V;
+        const VT *src = Sv + (size_t)kc * ncol / V;
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            int e = tid + i * NT;
+            if (e < nv) r[i] = src[e];
         }
-        for (int l = lend; l < l0 + LT; l++) D[(size_t)k * LT + (l - l0)] = (T)0;
+    };
+    load_chunk(0);
+    // FP64 recursion, one lane per k; D[k][l-l0] = w_k d^l(beta_k)
+    for (int k = tid; k < N; k += NT) {
+        const double wk = a.rt.w[k];
+        T *Dk = D + (size_t)k * LT;
+        group_recursion(a.rt, a.s2, B, g, m0, n0, k, lend, [&](int l, double v) {
+            if (l >= l0) Dk[l - l0] = (T)(wk * v);
+        });
+        for (int l = max(lend, l0); l < l0 + LT; l++) Dk[l - l0] = (T)0;
     }
-    // phase 2: GEMM over k in chunks staged in LDS
     const int rows_per_thread = 4;
     const int ngroups = NT / ncol;
     const int col = tid % ncol, rg = tid / ncol;
-    const T *S = reinterpret_cast<const T *>(a.src) + (size_t)g * N * C * np * 2;
-    constexpr int KC = 64;
     T acc[rows_per_thread] = {};
     for (int kc = 0; kc < N; kc += KC) {
         const int kn = min(KC, N - kc);
         __syncthreads();
-        for (int e = tid; e < kn * ncol; e += NT) Ss[e] = S[(size_t)kc * ncol + e];
+        {
+            const int nv = kn * ncol / V;
+            VT *dst = reinterpret_cast<VT *>(Ss);
+#pragma unroll
+            for (int i = 0; i < PF; i++) {
+                int e = tid + i * NT;
+                if (e < nv) dst[e] = r[i];
+            }
+        }
         __syncthreads();
+        if (kc + KC < N) load_chunk(kc + KC);
         if (rg < ngroups) {
             for (int kk = 0; kk < kn; kk++) {
                 T s = Ss[kk * ncol + col];
                 const T *dk = D + (size_t)(kc + kk) * LT + rg * rows_per_thread;
 #pragma unroll
-                for (int r = 0; r < rows_per_thread; r++) acc[r] += dk[r] * s;
+                for (int q = 0; q < rows_per_thread; q++) acc[q] += dk[q] * s;
             }
         }
     }
@@ -474,21 +600,24 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
     if (!valid) return;
     T *out = reinterpret_cast<T *>(a.dst) + 2 * ca.coef_off[ca.fn[fi]];
 #pragma unroll
-    for (int r = 0; r < rows_per_thread; r++) {
-        int l = l0 + rg * rows_per_thread + r;
+    for (int q = 0; q < rows_per_thread; q++) {
+        int l = l0 + rg * rows_per_thread + q;
         if (l < lend) {
             long idx = a.s2 ? s2_idx(l, m) : so3_idx(l, m, n);
-            out[2 * idx + ri] = (T)sp * acc[r];
+            out[2 * idx + ri] = (T)sp * acc[q];
         }
     }
 }
 
 // Inverse: G_k(pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) F^l(pair), c_l = 2l+1 (SO(3)) or 1 (S^2),
-// k in [k_start, k_start+KT).  Block = 32*C threads: 8C real columns x 4 k groups.
+// k in [k_start, k_start+KT).  Block = 32*C threads: ncc complex columns x k groups of 4.
+// The coefficient gather (scattered complex loads) is issued before the FP64 recursion.
 template <typename T, int C>
 __global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
 {
+    typedef typename cplx_t<T>::type T2;
     constexpr int NT = 32 * C;
+    constexpr int PF = 16;                      // gathered complex values per thread per pass
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, KT = a.tile;
@@ -497,69 +626,79 @@ __global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
     const int2 it = a.items[unit];
     const int g = it.x, k0 = it.y;
     const int np = a.s2 ? 2 : 4;
-    const int ncol = 2 * np * C;
+    const int ncc = np * C;                     // complex columns (fi, p)
     int m0, n0;
     if (a.s2) { m0 = g; n0 = 0; } else so3_group(g, m0, n0);
     const int M = B - m0;
     const int kn = min(KT, N - k0);
-    T *D = reinterpret_cast<T *>(smem_raw);     // [l-m0][KT]
-    T *Fv = D + (size_t)B * KT;                 // [l-m0][ncol]
+    T *D = reinterpret_cast<T *>(smem_raw);                          // [l-m0][KT]
+    T2 *Fv = reinterpret_cast<T2 *>(D + (size_t)B * KT);             // [l-m0][ncc]
     const int tid = threadIdx.x;
+    const int total = M * ncc;
+    T2 r[PF];
+    auto gather = [&](int base) {
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            int e = base + tid + i * NT;
+            if (e < total) {
+                int lr = e / ncc, cc = e - lr * ncc, l = m0 + lr;
+                int p = cc % np, fi = cc / np;
+                T2 val = cmk<T2>((T)0, (T)0);
+                if (fi < ca.nfc) {
+                    int m, n, sp;
+                    if (a.s2) { m = p ? -m0 : m0; n = 0; sp = p ? parity_sign(m0) : 1; }
+                    else { so3_pair(m0, n0, p, m, n); sp = so3_pair_sign(m0, n0, p); }
+                    const T2 *F = reinterpret_cast<const T2 *>(a.src) + ca.coef_off[ca.fn[fi]];
+                    T2 f = F[a.s2 ? s2_idx(l, m) : so3_idx(l, m, n)];
+                    val = cmk<T2>((T)sp * f.x, (T)sp * f.y);
+                }
+                r[i] = val;
+            }
+        }
+    };
+    auto deposit = [&](int base) {
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            int e = base + tid + i * NT;
+            if (e < total) Fv[e] = r[i];
+        }
+    };
+    gather(0);
     for (int kk = tid; kk < KT; kk += NT) {
-        int k = k0 + kk;
         if (kk >= kn) {
             for (int l = m0; l < B; l++) D[(size_t)(l - m0) * KT + kk] = (T)0;
             continue;
         }
-        if (a.s2) {
-            LegendreRec r;
-            r.init(a.rt, B, g, k);
-            double v = r.d1;
-            for (int l = m0; l < B; l++) {
-                if (l > m0) v = r.step(l);
-                D[(size_t)(l - m0) * KT + kk] = (T)v;
-            }
-        } else {
-            WignerRec r;
-            r.init(a.rt, B, g, m0, n0, k);
-            double v = r.d1;
-            for (int l = m0; l < B; l++) {
-                if (l > m0) v = r.step(l);
-                D[(size_t)(l - m0) * KT + kk] = (T)((double)(2 * l + 1) * v);
-            }
-        }
+        group_recursion(a.rt, a.s2, B, g, m0, n0, k0 + kk, B, [&](int l, double v) {
+            D[(size_t)(l - m0) * KT + kk] = (T)(a.s2 ? v : (double)(2 * l + 1) * v);
+        });
     }
-    // stage the coefficients of the group's pairs (sign folded in)
-    for (int e = tid; e < M * ncol; e += NT) {
-        int lr = e / ncol, col = e - lr * ncol, l = m0 + lr;
-        int ri = col & 1, p = (col >> 1) % np, fi = col / (2 * np);
-        T val = (T)0;
-        if (fi < ca.nfc) {
-            int m, n, sp;
-            if (a.s2) { m = p ? -m0 : m0; n = 0; sp = p ? parity_sign(m0) : 1; }
-            else { so3_pair(m0, n0, p, m, n); sp = so3_pair_sign(m0, n0, p); }
-            const T *F = reinterpret_cast<const T *>(a.src) + 2 * ca.coef_off[ca.fn[fi]];
-            long idx = a.s2 ? s2_idx(l, m) : so3_idx(l, m, n);
-            val = (T)sp * F[2 * idx + ri];
-        }
-        Fv[e] = val;
+    deposit(0);
+    for (int base = PF * NT; base < total; base += PF * NT) {
+        gather(base);
+        deposit(base);
     }
     __syncthreads();
-    const int ngroups = NT / ncol;
-    const int col = tid % ncol, kg = tid / ncol;
+    const int ngroups = NT / ncc;
+    const int cc = tid % ncc, kg = tid / ncc;
     if (kg >= ngroups) return;
-    T *G = reinterpret_cast<T *>(a.dst) + (size_t)g * N * C * np * 2;
+    T2 *G = reinterpret_cast<T2 *>(a.dst) + (size_t)g * N * ncc;
     for (int kb = kg * 4; kb < kn; kb += ngroups * 4) {
-        T acc[4] = {};
+        T2 acc[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] = cmk<T2>((T)0, (T)0);
         for (int lr = 0; lr < M; lr++) {
-            T f = Fv[lr * ncol + col];
+            T2 f = Fv[lr * ncc + cc];
             const T *dl = D + (size_t)lr * KT + kb;
 #pragma unroll
-            for (int j = 0; j < 4; j++) acc[j] += dl[j] * f;
+            for (int j = 0; j < 4; j++) {
+                acc[j].x += dl[j] * f.x;
+                acc[j].y += dl[j] * f.y;
+            }
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            if (kb + j < kn) G[(size_t)(k0 + kb + j) * ncol + col] = acc[j];
+            if (kb + j < kn) G[(size_t)(k0 + kb + j) * ncc + cc] = acc[j];
     }
 }
 

@@ -201,8 +201,11 @@ void or_wigner_d_stable(double beta, int L, double *d)
     free(col);
 }
 
-/* The oracle's transforms use the reference recursion where the reference is valid (B <= 64)
- * and the stable recursion beyond. */
+/* The oracle's grid transforms use the reference recursion where the reference is valid on the
+ * grid (B <= 64; it agrees with the stable recursion to ~1e-13 there) and the stable recursion
+ * beyond.  Pointwise evaluations (any beta) always use the stable recursion: the reference's seed
+ * step (68) divides by (l cos(beta) - n - 1), which vanishes at beta = pi/2 (n = -1) and loses
+ * all accuracy near it. */
 static void wigner_d(double beta, int L, double *d)
 {
     if (L <= 63)
@@ -430,7 +433,7 @@ void or_so3_inverse_point(int B, const double *Fd, double alpha, double beta, do
     const cplx *F = (const cplx *)Fd;
     int L = B - 1;
     double *d = (double *)malloc(sizeof(double) * (size_t)or_so3_ncoef(B));
-    wigner_d(beta, L, d);
+    or_wigner_d_stable(beta, L, d);
     cplx f = 0;
     for (int m = -L; m <= L; m++) {
         cplx ea = cexp(-I * (double)m * alpha);
@@ -682,7 +685,7 @@ void or_so3_real_harmonics(int L, double alpha, double beta, double gamma, doubl
 {
     long nc = or_so3_ncoef(L + 1);
     double *d = (double *)malloc(sizeof(double) * (size_t)nc);
-    wigner_d(beta, L, d);
+    or_wigner_d_stable(beta, L, d);
     memset(U, 0, sizeof(double) * (size_t)nc);
     for (int l = 0; l <= L; l++) {
         U[IX(l, 0, 0)] = d[IX(l, 0, 0)];
