@@ -1,5 +1,8 @@
 // api.hip -- plans, chunked execution and the C ABI of libfftso3_amd.so (gfx950).
 #include "kernels.hpp"
+#include "mfma.hpp"
+#include "fft128.hpp"
+#include "mfma32.hpp"
 
 #include "../../include/fftso3_amd.h"
 
@@ -76,11 +79,17 @@ struct Class {
     DevBuf d_tw32, d_tw64, d_rev;   // twiddles and DIF digit reversal
     FftDesc fd{};
     // recursion tables
-    DevBuf d_cb, d_lnc, d_lns, d_w, d_lnbin, d_sq, d_isq;
+    DevBuf d_cb, d_lnc, d_lns, d_w, d_lnbin, d_sq, d_isq, d_lcoef;
     DevBuf d_items_fwd, d_items_inv;
     int nitems_fwd = 0, nitems_inv = 0, lt_fwd = 0, kt_inv = 0;
+    DevBuf d_items_fwd_mx, d_items_inv_mx;   // MFMA kernels: 64-row and 128-k tiles
+    int nitems_fwd_mx = 0, nitems_inv_mx = 0;
     bool fused = false;             // whole slice fits in LDS
     int row_tile = 0, col_tile = 0; // two-pass fallback tiles
+    bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
+    int chunk = 8;                  // functions per pipeline pass for this class
+    DevBuf d_items_mx;              // (g, first function of a block of MX_FPI)
+    int nitems_mx = 0;
 };
 
 }  // namespace
@@ -94,6 +103,7 @@ struct fftso3_plan_s {
     long grid_total = 0, coef_total = 0;
     DevBuf d_grid_off, d_coef_off;
     std::vector<Class *> classes;
+    bool force_chunked = false;        // set_chunk() selects the chunked FP32 kernels
     DevBuf d_ws;        // group-major spectrum workspace
     DevBuf d_scratch;   // two-pass FFT scratch
     DevBuf d_fc;        // complex coefficients of the real path
@@ -223,6 +233,13 @@ int build_class_tables(fftso3_plan_s *pl, Class *c)
             lnbin[m] = (double)(0.5L * (logl((2.0L * m + 1.0L) / (4.0L * PI)) + lp));
         }
     }
+    std::vector<double> lcoef((size_t)3 * B, 0.0);
+    for (int l = 2; l < B; l++) {
+        lcoef[3 * l] = (double)l * (2.0 * l - 1.0);
+        lcoef[3 * l + 1] = (double)((2.0L * l - 1.0L) / (long double)(l - 1));
+        lcoef[3 * l + 2] = (double)((long double)l / (long double)(l - 1));
+    }
+    if ((st = c->d_lcoef.upload(lcoef))) return st;
     if ((st = c->d_cb.upload(cb)) || (st = c->d_lnc.upload(lnc)) || (st = c->d_lns.upload(lns)) ||
         (st = c->d_w.upload(w)) || (st = c->d_lnbin.upload(lnbin)) || (st = c->d_sq.upload(sq)) ||
         (st = c->d_isq.upload(isq)))
@@ -232,7 +249,10 @@ int build_class_tables(fftso3_plan_s *pl, Class *c)
 
 int build_class_work(fftso3_plan_s *pl, Class *c)
 {
-    const int B = c->B, N = c->N, C = pl->C;
+    const int B = c->B, N = c->N;
+    c->mx = pl->prec == FFTSO3_FP32 && N <= 128 && !pl->force_chunked;
+    c->chunk = c->mx ? (int)c->fns.size() : pl->C;
+    const int C = c->mx ? 8 : c->chunk;  // tiling of the chunked kernels (unused by the MX path)
     const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
     const size_t tsz = pl->esz / 2;  // real element size
     int st;
@@ -257,11 +277,27 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     c->nitems_fwd = (int)itf.size();
     c->nitems_inv = (int)iti.size();
     if ((st = c->d_items_fwd.upload(itf)) || (st = c->d_items_inv.upload(iti))) return st;
-    std::vector<int> ident(C);
-    for (int i = 0; i < C; i++) ident[i] = i;
-    std::vector<long> wsoff(C);
+    std::vector<int2> mf, mi;
+    for (int g = 0; g < ngrp; g++) {
+        int m0 = s2 ? g : (int)std::sqrt((double)g + 0.5);
+        if (!s2) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
+        for (int l = m0; l < B; l += 64) mf.push_back(make_int2(g, l));
+        for (int k = 0; k < N; k += 128) mi.push_back(make_int2(g, k));
+    }
+    c->nitems_fwd_mx = (int)mf.size();
+    c->nitems_inv_mx = (int)mi.size();
+    if ((st = c->d_items_fwd_mx.upload(mf)) || (st = c->d_items_inv_mx.upload(mi))) return st;
+    std::vector<int2> mx;
+    for (int g = 0; g < ngrp; g++)
+        for (int f = 0; f < (int)c->fns.size(); f += MX_FPI) mx.push_back(make_int2(g, f));
+    c->nitems_mx = (int)mx.size();
+    if ((st = c->d_items_mx.upload(mx))) return st;
+    const int CC = c->chunk;
+    std::vector<int> ident(CC);
+    for (int i = 0; i < CC; i++) ident[i] = i;
+    std::vector<long> wsoff(CC);
     const long nc = s2 ? (long)B * B : so3_ncoef(B);
-    for (int i = 0; i < C; i++) wsoff[i] = (long)i * nc;
+    for (int i = 0; i < CC; i++) wsoff[i] = (long)i * nc;
     if ((st = c->d_ident.upload(ident)) || (st = c->d_ws_off.upload(wsoff))) return st;
     // FFT mode
     const size_t tables = (size_t)N * (pl->esz + sizeof(int));
@@ -298,7 +334,7 @@ int finalize_plan(fftso3_plan_s *pl)
     for (auto *c : pl->classes) {
         if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
         if ((st = c->d_fn.upload(c->fns))) return st;
-        const size_t B = c->B, N = c->N, C = pl->C;
+        const size_t B = c->B, N = c->N, C = c->chunk;
         size_t w = s2 ? C * B * N * 2 * pl->esz : C * B * B * N * 4 * pl->esz;
         ws = std::max(ws, w);
         if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
@@ -364,7 +400,7 @@ ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
     ChunkArgs ca;
     ca.B = c->B;
     ca.N = c->N;
-    ca.C = pl->C;
+    ca.C = c->chunk;
     ca.nfc = nfc;
     ca.fn = reinterpret_cast<const int *>(c->d_fn.p) + i0;
     ca.grid_off = reinterpret_cast<const long *>(pl->d_grid_off.p);
@@ -398,6 +434,7 @@ RecTables rec_tables(Class *c)
     rt.lnbin = (const double *)c->d_lnbin.p;
     rt.sq = (const double *)c->d_sq.p;
     rt.isq = (const double *)c->d_isq.p;
+    rt.lcoef = (const double *)c->d_lcoef.p;
     return rt;
 }
 
@@ -416,7 +453,8 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
         a.items = (const int2 *)c->d_items_fwd.p;
         a.nitems = c->nitems_fwd;
         a.tile = c->lt_fwd;
-        size_t lds = ((size_t)c->N * a.tile + 64 * (size_t)ncol) * sizeof(T);
+        // D[N][LT] + spectrum chunk [min(N,128)][ncol] (contract_fwd's KCMAX)
+        size_t lds = ((size_t)c->N * a.tile + (size_t)std::min(c->N, 128) * ncol) * sizeof(T);
         hipLaunchKernelGGL((contract_fwd<T, C>), dim3(xcd_grid(a.nitems)), dim3(32 * C), lds, s, a);
     } else {
         a.items = (const int2 *)c->d_items_inv.p;
@@ -429,16 +467,77 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
     return FFTSO3_OK;
 }
 
+template <int NP, int NPC>
+int launch_contract_bf16(Class *c, const ConArgs<float> &a0, bool fwd, hipStream_t s)
+{
+    ConArgs<float> a = a0;
+    if (fwd) {
+        a.items = (const int2 *)c->d_items_fwd_mx.p;
+        a.nitems = c->nitems_fwd_mx;
+        a.tile = 64;
+        size_t lds = (size_t)NPC * (64 + 16 * NP) * 72 * 2;
+        hipLaunchKernelGGL((contract_fwd_bf16<NP, NPC>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+    } else {
+        a.items = (const int2 *)c->d_items_inv_mx.p;
+        a.nitems = c->nitems_inv_mx;
+        a.tile = 128;
+        size_t lds = (size_t)NPC * (128 + 16 * NP) * 72 * 2;
+        hipLaunchKernelGGL((contract_inv_bf16<NP, NPC>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
+template <int NP>
+int launch_contract_mx(Class *c, const ConArgs<float> &a0, bool fwd, hipStream_t s)
+{
+    ConArgs<float> a = a0;
+    a.items = (const int2 *)c->d_items_mx.p;
+    a.nitems = c->nitems_mx;
+    a.tile = MX_FPI;
+    const size_t lds = (size_t)64 * MX_DPITCH * sizeof(float);
+    if (fwd) hipLaunchKernelGGL((contract_fwd_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
 template <typename T>
 int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
 {
-    switch (pl->C) {
+    if constexpr (sizeof(T) == 4) {
+        if (c->mx) {
+            ConArgs<float> a;
+            a.ca = ca;
+            a.rt = rec_tables(c);
+            a.s2 = pl->domain == FFTSO3_DOMAIN_S2;
+            a.src = src;
+            a.dst = dst;
+            return a.s2 ? launch_contract_mx<2>(c, a, fwd, s) : launch_contract_mx<4>(c, a, fwd, s);
+        }
+        if (pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6) {
+            ConArgs<float> a;
+            a.ca = ca;
+            a.rt = rec_tables(c);
+            a.s2 = pl->domain == FFTSO3_DOMAIN_S2;
+            a.src = src;
+            a.dst = dst;
+            const bool x6 = pl->prec == FFTSO3_BF16X6;
+            if (a.s2) return x6 ? launch_contract_bf16<2, 3>(c, a, fwd, s) : launch_contract_bf16<2, 2>(c, a, fwd, s);
+            return x6 ? launch_contract_bf16<4, 3>(c, a, fwd, s) : launch_contract_bf16<4, 2>(c, a, fwd, s);
+        }
+    }
+    switch (c->chunk) {
     case 1: return launch_contract<T, 1>(pl, c, ca, fwd, src, dst, s);
     case 2: return launch_contract<T, 2>(pl, c, ca, fwd, src, dst, s);
     case 4: return launch_contract<T, 4>(pl, c, ca, fwd, src, dst, s);
     default: return launch_contract<T, 8>(pl, c, ca, fwd, src, dst, s);
     }
 }
+
+bool g_disable_fft128 = false;
+
+template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128 && !g_disable_fft128; }
 
 template <typename T>
 int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void *in, void *out, hipStream_t s)
@@ -468,15 +567,16 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                 size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((s2_fft_fwd<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((s2_fft_fwd<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            } else if (use_fft128<T>(c)) {
+                if constexpr (sizeof(T) == 4) {
+                    size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
+                    if (real) hipLaunchKernelGGL((so3_fft128_fwd<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                    else hipLaunchKernelGGL((so3_fft128_fwd<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                }
             } else if (c->fused) {
                 size_t lds = (size_t)N * N * sizeof(T2) + tables;
-                if (real) {
-                    auto k = so3_fft_fwd_slice<T, true>;
-                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
-                } else {
-                    auto k = so3_fft_fwd_slice<T, false>;
-                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
-                }
+                if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
             } else {
                 a.tile = c->row_tile;
                 int nunit = nfc * N * (N / a.tile);
@@ -523,15 +623,16 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                 size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((s2_fft_inv<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((s2_fft_inv<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            } else if (use_fft128<T>(c)) {
+                if constexpr (sizeof(T) == 4) {
+                    size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
+                    if (real) hipLaunchKernelGGL((so3_fft128_inv<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                    else hipLaunchKernelGGL((so3_fft128_inv<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                }
             } else if (c->fused) {
                 size_t lds = (size_t)N * N * sizeof(T2) + tables;
-                if (real) {
-                    auto k = so3_fft_inv_slice<T, true>;
-                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
-                } else {
-                    auto k = so3_fft_inv_slice<T, false>;
-                    hipLaunchKernelGGL(k, dim3(persistent_grid(k, lds, nfc * N)), dim3(kFftThreads), lds, s, a);
-                }
+                if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
             } else {
                 a.tile = c->col_tile;
                 int nunit = nfc * N * (N / a.tile);
@@ -556,8 +657,8 @@ int run(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStream_t s)
     int st;
     for (auto *c : pl->classes) {
         const int n = (int)c->fns.size();
-        for (int i0 = 0; i0 < n; i0 += pl->C)
-            if ((st = run_chunk<T>(pl, c, i0, std::min(pl->C, n - i0), fwd, in, out, s))) return st;
+        for (int i0 = 0; i0 < n; i0 += c->chunk)
+            if ((st = run_chunk<T>(pl, c, i0, std::min(c->chunk, n - i0), fwd, in, out, s))) return st;
     }
     return FFTSO3_OK;
 }
@@ -611,6 +712,22 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv<double, 2>));
     ALLOW_LDS((contract_inv<double, 4>));
     ALLOW_LDS((contract_inv<double, 8>));
+    ALLOW_LDS((contract_fwd_mx32<4>));
+    ALLOW_LDS((contract_fwd_mx32<2>));
+    ALLOW_LDS((contract_inv_mx32<4>));
+    ALLOW_LDS((contract_inv_mx32<2>));
+    ALLOW_LDS((so3_fft128_fwd<true>));
+    ALLOW_LDS((so3_fft128_fwd<false>));
+    ALLOW_LDS((so3_fft128_inv<true>));
+    ALLOW_LDS((so3_fft128_inv<false>));
+    ALLOW_LDS((contract_fwd_bf16<4, 2>));
+    ALLOW_LDS((contract_fwd_bf16<4, 3>));
+    ALLOW_LDS((contract_fwd_bf16<2, 2>));
+    ALLOW_LDS((contract_fwd_bf16<2, 3>));
+    ALLOW_LDS((contract_inv_bf16<4, 2>));
+    ALLOW_LDS((contract_inv_bf16<4, 3>));
+    ALLOW_LDS((contract_inv_bf16<2, 2>));
+    ALLOW_LDS((contract_inv_bf16<2, 3>));
 #undef ALLOW_LDS
     kernel_attrs_done = true;
     return FFTSO3_OK;
@@ -658,6 +775,7 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
     }
     pl->C = 1;
     while (pl->C < 8 && pl->C < maxfn) pl->C *= 2;
+    if (precision == FFTSO3_BF16X3 || precision == FFTSO3_BF16X6) pl->C = 8;  // MFMA tiles span 8 functions
     if ((st = pl->d_grid_off.upload(pl->grid_off)) || (st = pl->d_coef_off.upload(pl->coef_off)) ||
         (st = finalize_plan(pl))) {
         delete pl;
@@ -702,15 +820,18 @@ int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk)
     if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
     if (chunk != 1 && chunk != 2 && chunk != 4 && chunk != 8)
         return fail(FFTSO3_EINVAL, "chunk must be 1, 2, 4 or 8 (got %d)", chunk);
-    if (chunk == plan->C) return FFTSO3_OK;
+    if ((plan->prec == FFTSO3_BF16X3 || plan->prec == FFTSO3_BF16X6) && chunk != 8)
+        return fail(FFTSO3_EINVAL, "split-BF16 plans process 8 functions per chunk");
+    if (chunk == plan->C && plan->force_chunked) return FFTSO3_OK;
     plan->C = chunk;
+    plan->force_chunked = true;
     return finalize_plan(plan);
 }
 
 int fftso3_plan_get_chunk(fftso3_plan_t plan, int *chunk)
 {
     if (!plan || !chunk) return fail(FFTSO3_EINVAL, "NULL argument");
-    *chunk = plan->C;
+    *chunk = plan->classes.empty() ? plan->C : plan->classes[0]->chunk;
     return FFTSO3_OK;
 }
 
@@ -739,8 +860,7 @@ static int run_dispatch(fftso3_plan_t plan, bool fwd, const void *in, void *out,
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     switch (plan->prec) {
     case FFTSO3_FP64: return run<double>(plan, fwd, in, out, s);
-    case FFTSO3_FP32: return run<float>(plan, fwd, in, out, s);
-    default: return fail(FFTSO3_EUNSUPPORTED, "precision %d not implemented", plan->prec);
+    default: return run<float>(plan, fwd, in, out, s);
     }
 }
 

@@ -1,0 +1,236 @@
+// fft128.hpp -- 2-D DFT of one 128 x 128 complex-float beta slice (the l_max = 63 grid), gfx950.
+//
+// 128 = 8 x 16: stage A is a radix-8 butterfly over elements j + 16 r (r < 8) followed by the
+// twiddle W_128^(j s), written back in place; stage B is a 16-point DFT of the 16 contiguous
+// results y[16 s + j], done entirely in registers, which leaves X[s + 8 q] in natural order (a
+// barrier separates stage B's reads from its strided writes, which land in other threads' input).  Rows (gamma) first, then
+// columns (alpha).  The slice lives in LDS with a row pitch of 129 elements so that every stage's
+// lane pattern is conflict free; the forward kernel's row stage A reads the grid straight from
+// HBM.  One workgroup of 1024 threads per slice.
+#pragma once
+
+#include "kernels.hpp"
+
+#define F128 128
+#define P128 129  // LDS row pitch (elements)
+
+// 16-point DFT in registers: v[q] <- sum_j v[j] W16^(j q), as 4 x 4 (radix-4, twiddle, radix-4)
+__device__ __forceinline__ void dft16(float2 *v)
+{
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+    // stage 1: four radix-4 DFTs over j = j1 + 4 j2 (j2 summed), index j1
+    float2 a[16];
+#pragma unroll
+    for (int j1 = 0; j1 < 4; j1++) {
+        float2 t[4] = {v[j1], v[j1 + 4], v[j1 + 8], v[j1 + 12]};
+        dft4(t);
+#pragma unroll
+        for (int q2 = 0; q2 < 4; q2++) a[j1 * 4 + q2] = t[q2];  // a[j1][q2]
+    }
+    // twiddle W16^(j1 q2)
+    const float2 w[10] = {
+        {1.f, 0.f},  {c1, -s1}, {h, -h},   {s1, -c1},  // W^0..W^3
+        {0.f, -1.f}, {-s1, -c1}, {-h, -h}, {-c1, -s1}, // W^4..W^7
+        {-1.f, 0.f}, {-c1, s1}                          // W^8, W^9
+    };
+#pragma unroll
+    for (int j1 = 1; j1 < 4; j1++)
+#pragma unroll
+        for (int q2 = 1; q2 < 4; q2++) a[j1 * 4 + q2] = cmul(a[j1 * 4 + q2], w[j1 * q2]);
+    // stage 2: radix-4 over j1 for each q2; q = q2 + 4 q1
+#pragma unroll
+    for (int q2 = 0; q2 < 4; q2++) {
+        float2 t[4] = {a[q2], a[4 + q2], a[8 + q2], a[12 + q2]};
+        dft4(t);
+#pragma unroll
+        for (int q1 = 0; q1 < 4; q1++) v[q2 + 4 * q1] = t[q1];
+    }
+}
+
+// radix-8 stage A on 8 values x[j + 16 r], then twiddle W128^(j s) from the LDS table
+__device__ __forceinline__ void stageA(float2 *v, int j, const float2 *twl)
+{
+    dft8(v);
+#pragma unroll
+    for (int s = 1; s < 8; s++) v[s] = cmul(v[s], twl[(s * j) & 127]);
+}
+
+// Row pass over a slice already in LDS (stage A reads LDS).
+__device__ __forceinline__ void rows_from_lds(float2 *buf, const float2 *twl, int tid)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int t = tid + i * 1024, row = t >> 4, j = t & 15;
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = buf[row * P128 + j + 16 * r];
+        stageA(v, j, twl);
+#pragma unroll
+        for (int s = 0; s < 8; s++) buf[row * P128 + j + 16 * s] = v[s];
+    }
+}
+
+__device__ __forceinline__ void rows_stageB(float2 *buf, int tid)
+{
+    const int row = tid & 127, s = tid >> 7;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = buf[row * P128 + 16 * s + q];
+    __syncthreads();  // the strided write-back below overlaps other threads' contiguous reads
+    dft16(v);
+#pragma unroll
+    for (int q = 0; q < 16; q++) buf[row * P128 + s + 8 * q] = v[q];
+}
+
+__device__ __forceinline__ void cols_pass(float2 *buf, const float2 *twl, int tid)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int t = tid + i * 1024, col = t & 127, j = t >> 7;
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = buf[(j + 16 * r) * P128 + col];
+        stageA(v, j, twl);
+#pragma unroll
+        for (int s = 0; s < 8; s++) buf[(j + 16 * s) * P128 + col] = v[s];
+    }
+    __syncthreads();
+    {
+        const int col = tid & 127, s = tid >> 7;
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = buf[(16 * s + q) * P128 + col];
+        __syncthreads();  // as in rows_stageB: reads and strided writes overlap across threads
+        dft16(v);
+#pragma unroll
+        for (int q = 0; q < 16; q++) buf[(s + 8 * q) * P128 + col] = v[q];
+    }
+    __syncthreads();
+}
+
+template <bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft128_fwd(FftArgs<float> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *buf = reinterpret_cast<float2 *>(smem_raw);
+    float2 *twl = buf + F128 * P128;
+    const ChunkArgs &ca = a.ca;
+    const int nslice = ca.nfc * F128;
+    const int unit = xcd_unit(nslice);
+    if (unit >= nslice) return;
+    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+    const int tid = threadIdx.x;
+    if (tid < F128) twl[tid] = a.tw[tid];
+    const long base = ca.grid_off[ca.fn[fi]] + (long)k * F128 * F128;
+    // row stage A straight from HBM: 2 tasks x 8 loads in flight per thread
+    float2 v[2][8];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int t = tid + i * 1024, row = t >> 4, j = t & 15;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const long e = base + row * F128 + j + 16 * r;
+            if (REAL) v[i][r] = make_float2(reinterpret_cast<const float *>(a.src)[e], 0.f);
+            else v[i][r] = reinterpret_cast<const float2 *>(a.src)[e];
+        }
+    }
+    __syncthreads();  // twiddle table
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int t = tid + i * 1024, row = t >> 4, j = t & 15;
+        stageA(v[i], j, twl);
+#pragma unroll
+        for (int s = 0; s < 8; s++) buf[row * P128 + j + 16 * s] = v[i][s];
+    }
+    __syncthreads();
+    rows_stageB(buf, tid);
+    __syncthreads();
+    cols_pass(buf, twl, tid);
+    // emit S_k(m,n) = X(-m mod 128, -n mod 128) for the 4 pairs of every group: 32-byte stores
+    float4 *S4 = reinterpret_cast<float4 *>(a.dst);
+    const int B = 64;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int g = tid + i * 1024;  // 4096 groups
+        int m0, n0;
+        so3_group(g, m0, n0);
+        const int um = (-m0) & 127, un = (-n0) & 127, vm = m0 & 127, vn = n0 & 127;
+        float2 p0 = buf[um * P128 + un];     // (m0, n0)
+        float2 p1 = buf[vm * P128 + vn];     // (-m0, -n0)
+        float2 p2 = buf[vn * P128 + vm];     // (-n0, -m0)
+        float2 p3 = buf[un * P128 + um];     // (n0, m0)
+        const long o = (((long)g * F128 + k) * ca.C + fi) * 2;
+        S4[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
+        S4[o + 1] = make_float4(p2.x, p2.y, p3.x, p3.y);
+    }
+    (void)B;
+}
+
+template <bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft128_inv(FftArgs<float> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *buf = reinterpret_cast<float2 *>(smem_raw);
+    float2 *twl = buf + F128 * P128;
+    const ChunkArgs &ca = a.ca;
+    const int nslice = ca.nfc * F128;
+    const int unit = xcd_unit(nslice);
+    if (unit >= nslice) return;
+    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
+    const int tid = threadIdx.x;
+    if (tid < F128) twl[tid] = a.tw[tid];
+    // gather the 4 pairs of every group (32-byte loads), scatter to bins (m mod 128, n mod 128)
+    const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
+    float4 q[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int g = tid + i * 1024;
+        const long o = (((long)g * F128 + k) * ca.C + fi) * 2;
+        q[i][0] = G4[o];
+        q[i][1] = G4[o + 1];
+    }
+    // Nyquist row and column (m or n = +-64) are zero
+    if (tid < F128) {
+        buf[64 * P128 + tid] = make_float2(0.f, 0.f);
+        buf[tid * P128 + 64] = make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int g = tid + i * 1024;
+        int m0, n0;
+        so3_group(g, m0, n0);
+        const int vm = m0 & 127, vn = n0 & 127, um = (-m0) & 127, un = (-n0) & 127;
+        buf[vm * P128 + vn] = make_float2(q[i][0].x, q[i][0].y);                    // (m0, n0)
+        if (m0 > 0) {
+            buf[um * P128 + un] = make_float2(q[i][0].z, q[i][0].w);                // (-m0, -n0)
+            if (n0 != m0 && n0 != -m0) {
+                buf[un * P128 + um] = make_float2(q[i][1].x, q[i][1].y);            // (-n0, -m0)
+                buf[vn * P128 + vm] = make_float2(q[i][1].z, q[i][1].w);            // (n0, m0)
+            }
+        }
+    }
+    __syncthreads();
+    rows_from_lds(buf, twl, tid);
+    __syncthreads();
+    rows_stageB(buf, tid);
+    __syncthreads();
+    cols_pass(buf, twl, tid);
+    const long base = ca.grid_off[ca.fn[fi]] + (long)k * F128 * F128;
+    if (REAL) {
+        float *out = reinterpret_cast<float *>(a.dst) + base;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int e4 = tid + i * 1024, row = e4 >> 5, c4 = (e4 & 31) * 4;  // 4 reals per lane
+            const float2 *r = buf + row * P128 + c4;
+            reinterpret_cast<float4 *>(out)[e4] = make_float4(r[0].x, r[1].x, r[2].x, r[3].x);
+        }
+    } else {
+        float4 *out = reinterpret_cast<float4 *>(reinterpret_cast<float2 *>(a.dst) + base);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int e2 = tid + i * 1024, row = e2 >> 6, c2 = (e2 & 63) * 2;   // 2 complex per lane
+            const float2 *r = buf + row * P128 + c2;
+            out[e2] = make_float4(r[0].x, r[0].y, r[1].x, r[1].y);
+        }
+    }
+}

@@ -80,17 +80,7 @@ __device__ __forceinline__ void load_fft_tables(T2 *twl, int *revl, const T2 *tw
     }
 }
 
-// Persistent workgroups walk the (k, fi) slices of the chunk; the units that are in flight at
-// the same time on one XCD are consecutive (k, fi) pairs, so their workspace writes share L2
-// lines.  gridDim.x is a multiple of 8.
-__device__ __forceinline__ int persistent_first_unit()
-{
-    const int perx = gridDim.x / XCD_COUNT;
-    return (blockIdx.x % XCD_COUNT) * perx + blockIdx.x / XCD_COUNT;
-}
-
-// Fused forward: one (fi, k) slice at a time, whole N x N slice in LDS; the next slice is
-// prefetched into registers while the current one is transformed.
+// Fused forward: one (fi, k) slice per workgroup, whole N x N slice in LDS.
 template <typename T, bool REAL>
 __global__ void __launch_bounds__(1024) so3_fft_fwd_slice(FftArgs<T> a)
 {
@@ -101,50 +91,37 @@ __global__ void __launch_bounds__(1024) so3_fft_fwd_slice(FftArgs<T> a)
     T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     T2 *twl = buf + NN;
     int *revl = reinterpret_cast<int *>(twl + N);
+    const int unit = xcd_unit(nslice);
+    if (unit >= nslice) return;
+    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
     const int tid = threadIdx.x, nthr = blockDim.x;
     load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
-    T2 r[FFT_EPT];
-    auto prefetch = [&](int unit) {
-        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
-        const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
-#pragma unroll
-        for (int i = 0; i < FFT_EPT; i++) {
-            int e = tid + i * nthr;
-            if (e < NN) {
-                if (REAL) r[i] = cmk<T2>(reinterpret_cast<const T *>(a.src)[base + e], (T)0);
-                else r[i] = reinterpret_cast<const T2 *>(a.src)[base + e];
-            }
-        }
-    };
+    const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
+    if (REAL) {
+        const T *g = reinterpret_cast<const T *>(a.src) + base;
+        fill_lds<8>(buf, NN, tid, nthr, [&](int e) { return cmk<T2>(g[e], (T)0); });
+    } else {
+        const T2 *g = reinterpret_cast<const T2 *>(a.src) + base;
+        fill_lds<8>(buf, NN, tid, nthr, [&](int e) { return g[e]; });
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, N, true, twl, tid, nthr);   // over j2 (gamma)
+    fft_tile(buf, a.fd, N, 1, N, false, twl, tid, nthr);  // over j1 (alpha)
+    // emit S_k(m,n) = X(-m mod N, -n mod N) for the 4 pairs of every group
     T2 *S = reinterpret_cast<T2 *>(a.dst);
-    const int nq = 4 * B * B;
-    int unit = persistent_first_unit();
-    if (unit < nslice) prefetch(unit);
-    for (; unit < nslice; unit += gridDim.x) {
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < FFT_EPT; i++) {
-            int e = tid + i * nthr;
-            if (e < NN) buf[e] = r[i];
-        }
-        __syncthreads();
-        if (unit + (int)gridDim.x < nslice) prefetch(unit + gridDim.x);
-        fft_tile(buf, a.fd, 1, N, N, true, twl, tid, nthr);   // over j2 (gamma)
-        fft_tile(buf, a.fd, N, 1, N, false, twl, tid, nthr);  // over j1 (alpha)
-        // emit S_k(m,n) = X(-m mod N, -n mod N) to its group slot
-        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
-        for (int q = tid; q < nq; q += nthr) {
-            int g = q >> 2, p = q & 3, m0, n0, m, n;
-            so3_group(g, m0, n0);
-            so3_pair(m0, n0, p, m, n);
-            int u = revl[wrapN(-m, N)], v = revl[wrapN(-n, N)];
-            S[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[u * N + v];
-        }
+    for (int g = tid; g < B * B; g += nthr) {
+        int m0, n0;
+        so3_group(g, m0, n0);
+        const int um = revl[wrapN(-m0, N)], un = revl[wrapN(-n0, N)], vm = revl[wrapN(m0, N)], vn = revl[wrapN(n0, N)];
+        T2 *o = S + (((long)g * N + k) * ca.C + fi) * 4;
+        o[0] = buf[um * N + un];
+        o[1] = buf[vm * N + vn];
+        o[2] = buf[vn * N + vm];
+        o[3] = buf[un * N + um];
     }
 }
 
-// Fused inverse: gather G_k(m,n) into bins (m mod N, n mod N), 2-D DFT, store the slice; the
-// next slice's gather is issued before the current transform.
+// Fused inverse: gather G_k(m,n) into bins (m mod N, n mod N), 2-D DFT, store the slice.
 template <typename T, bool REAL>
 __global__ void __launch_bounds__(1024) so3_fft_inv_slice(FftArgs<T> a)
 {
@@ -155,48 +132,41 @@ __global__ void __launch_bounds__(1024) so3_fft_inv_slice(FftArgs<T> a)
     T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     T2 *twl = buf + NN;
     int *revl = reinterpret_cast<int *>(twl + N);
+    const int unit = xcd_unit(nslice);
+    if (unit >= nslice) return;
+    const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
     const int tid = threadIdx.x, nthr = blockDim.x;
     load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const T2 *G = reinterpret_cast<const T2 *>(a.src);
-    T2 r[FFT_EPT];
-    auto prefetch = [&](int unit) {
-        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
-#pragma unroll
-        for (int i = 0; i < FFT_EPT; i++) {
-            int e = tid + i * nthr;
-            if (e < NN) {
-                int u = e / N, v = e - u * N;
-                T2 val = cmk<T2>((T)0, (T)0);
-                if (u != B && v != B) {
-                    int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
-                    so3_slot(m, n, g, p);
-                    val = G[(((long)g * N + k) * ca.C + fi) * 4 + p];
-                }
-                r[i] = val;
+    // Nyquist row and column are zero; every other bin holds exactly one pair of one group
+    for (int i = tid; i < N; i += nthr) {
+        buf[B * N + i] = cmk<T2>((T)0, (T)0);
+        buf[i * N + B] = cmk<T2>((T)0, (T)0);
+    }
+    for (int g = tid; g < B * B; g += nthr) {
+        int m0, n0;
+        so3_group(g, m0, n0);
+        const T2 *q = G + (((long)g * N + k) * ca.C + fi) * 4;
+        T2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        const int vm = wrapN(m0, N), vn = wrapN(n0, N), um = wrapN(-m0, N), un = wrapN(-n0, N);
+        buf[vm * N + vn] = q0;
+        if (m0 > 0) {
+            buf[um * N + un] = q1;
+            if (n0 != m0 && n0 != -m0) {
+                buf[un * N + um] = q2;
+                buf[vn * N + vm] = q3;
             }
         }
-    };
-    int unit = persistent_first_unit();
-    if (unit < nslice) prefetch(unit);
-    for (; unit < nslice; unit += gridDim.x) {
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < FFT_EPT; i++) {
-            int e = tid + i * nthr;
-            if (e < NN) buf[e] = r[i];
-        }
-        __syncthreads();
-        if (unit + (int)gridDim.x < nslice) prefetch(unit + gridDim.x);
-        fft_tile(buf, a.fd, 1, N, N, true, twl, tid, nthr);
-        fft_tile(buf, a.fd, N, 1, N, false, twl, tid, nthr);
-        const int k = unit / ca.nfc, fi = unit - k * ca.nfc;
-        const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
-        for (int e = tid; e < NN; e += nthr) {
-            int j1 = e / N, j2 = e - j1 * N;
-            T2 val = buf[revl[j1] * N + revl[j2]];
-            if (REAL) reinterpret_cast<T *>(a.dst)[base + e] = val.x;
-            else reinterpret_cast<T2 *>(a.dst)[base + e] = val;
-        }
+    }
+    __syncthreads();
+    fft_tile(buf, a.fd, 1, N, N, true, twl, tid, nthr);
+    fft_tile(buf, a.fd, N, 1, N, false, twl, tid, nthr);
+    const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
+    for (int e = tid; e < NN; e += nthr) {
+        int j1 = e / N, j2 = e - j1 * N;
+        T2 val = buf[revl[j1] * N + revl[j2]];
+        if (REAL) reinterpret_cast<T *>(a.dst)[base + e] = val.x;
+        else reinterpret_cast<T2 *>(a.dst)[base + e] = val;
     }
 }
 
@@ -363,7 +333,7 @@ __global__ void __launch_bounds__(1024) s2_fft_inv(FftArgs<T> a)
     load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
     const T2 *G = reinterpret_cast<const T2 *>(a.src);
     // G_k(m) at bin (-m mod N)
-    fill_lds<8>(buf, R * N, tid, nthr, [&](int e) {
+    fill_lds<4>(buf, R * N, tid, nthr, [&](int e) {
         int r = e / N, u = e - r * N, k = rb * R + r;
         T2 val = cmk<T2>((T)0, (T)0);
         if (u != B) {
@@ -395,6 +365,7 @@ struct RecTables {
     const double *lnbin;   // SO(3): 0.5 ln C(2m0, m0+n0) per g  [B^2]; S^2: seed log per m0 [B]
     const double *sq;      // SO(3): sqrt(l^2 - m^2)             [B*B] (l-major); S^2: fact table
     const double *isq;     // SO(3): 1/sqrt(l^2 - m^2);          S^2: 1/fact
+    const double *lcoef;   // SO(3): per l {l(2l-1), (2l-1)/(l-1), l/(l-1)} [3B]
 };
 
 // Wigner d^l_{m0 n0}(beta_k) for l = m0, m0+1, ...: seed d^{m0}_{m0 n0} =
@@ -405,12 +376,12 @@ struct RecTables {
 struct WignerRec {
     int B, m0, n0, an0;
     double cb, d1, d2;  // d1 = d^{l-1}, d2 = d^{l-2}
-    const double *sq, *isq;
-    __device__ void init(const RecTables &t, int B_, int g, int m0_, int n0_, int k)
+    const double *sq, *isq, *lc;
+    __device__ __forceinline__ void init(const RecTables &t, int B_, int g, int m0_, int n0_, int k)
     {
         B = B_; m0 = m0_; n0 = n0_; an0 = iabs(n0_);
         cb = t.cb[k];
-        sq = t.sq; isq = t.isq;
+        sq = t.sq; isq = t.isq; lc = t.lcoef;
         int a = m0 + n0, b = m0 - n0;
         double x = t.lnbin[g] + (a ? a * t.lnc[k] : 0.0) + (b ? b * t.lns[k] : 0.0);
         double s = exp(x);
@@ -424,12 +395,10 @@ struct WignerRec {
         if (l == 1) {
             v = cb * d1;
         } else {
-            double dl = (double)l;
-            double ir = isq[l * B + m0] * isq[l * B + an0];
-            double inv_lm1 = 1.0 / (dl - 1.0);
-            double A = dl * (2.0 * dl - 1.0) * ir;
-            double Bc = (2.0 * dl - 1.0) * (double)(m0 * n0) * inv_lm1 * ir;
-            double Cc = dl * inv_lm1 * sq[(l - 1) * B + m0] * sq[(l - 1) * B + an0] * ir;
+            const double ir = isq[l * B + m0] * isq[l * B + an0];
+            const double A = lc[3 * l] * ir;
+            const double Bc = lc[3 * l + 1] * (double)(m0 * n0) * ir;
+            const double Cc = lc[3 * l + 2] * sq[(l - 1) * B + m0] * sq[(l - 1) * B + an0] * ir;
             v = (A * cb - Bc) * d1 - Cc * d2;
         }
         d2 = d1;
@@ -445,7 +414,7 @@ struct LegendreRec {
     int B, m0;
     double x, d1, d2;
     const double *fact, *ifact;
-    __device__ void init(const RecTables &t, int B_, int g, int k)
+    __device__ __forceinline__ void init(const RecTables &t, int B_, int g, int k)
     {
         B = B_; m0 = g;
         x = t.cb[k];
@@ -478,9 +447,13 @@ template <typename T> struct ConArgs {
     int s2;               // 1 for S^2 (two pairs, Legendre recursion)
 };
 
+// 16-byte native vectors (clang ext_vector_type; HIP's float4 class copies lower to memcpy and
+// would keep register arrays of them in scratch)
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef double nd2 __attribute__((ext_vector_type(2)));
 template <typename T> struct vec16;
-template <> struct vec16<float> { typedef float4 type; static constexpr int n = 4; };
-template <> struct vec16<double> { typedef double2 type; static constexpr int n = 2; };
+template <> struct vec16<float> { typedef nf4 type; static constexpr int n = 4; };
+template <> struct vec16<double> { typedef nd2 type; static constexpr int n = 2; };
 
 // Run the recursion of group g for one beta_k and hand every (l, value) with l in [m0, lend) to f.
 template <typename F>
@@ -537,16 +510,17 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
     const int tid = threadIdx.x;
     const VT *Sv = reinterpret_cast<const VT *>(reinterpret_cast<const T *>(a.src) + (size_t)g * N * ncol);
     VT r[PF];
-    auto load_chunk = [&](int kc) {
-        const int nv = min(KC, N - kc) * ncol / V;
-        const VT *src = Sv + (size_t)kc * ncol / V;
-#pragma unroll
-        for (int i = 0; i < PF; i++) {
-            int e = tid + i * NT;
-            if (e < nv) r[i] = src[e];
-        }
-    };
-    load_chunk(0);
+#define CONTRACT_FWD_LOAD(kc_)                                               \
+    {                                                                        \
+        const int nv_ = min(KC, N - (kc_)) * ncol / V;                       \
+        const VT *src_ = Sv + (size_t)(kc_) * ncol / V;                      \
+        _Pragma("unroll") for (int i = 0; i < PF; i++)                       \
+        {                                                                    \
+            int e = tid + i * NT;                                            \
+            if (e < nv_) r[i] = src_[e];                                     \
+        }                                                                    \
+    }
+    CONTRACT_FWD_LOAD(0)
     // FP64 recursion, one lane per k; D[k][l-l0] = w_k d^l(beta_k)
     for (int k = tid; k < N; k += NT) {
         const double wk = a.rt.w[k];
@@ -573,7 +547,7 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
             }
         }
         __syncthreads();
-        if (kc + KC < N) load_chunk(kc + KC);
+        if (kc + KC < N) CONTRACT_FWD_LOAD(kc + KC)
         if (rg < ngroups) {
             for (int kk = 0; kk < kn; kk++) {
                 T s = Ss[kk * ncol + col];
@@ -583,6 +557,7 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
             }
         }
     }
+#undef CONTRACT_FWD_LOAD
     if (rg >= ngroups) return;
     const int ri = col & 1, p = (col >> 1) % np, fi = col / (2 * np);
     if (fi >= ca.nfc) return;

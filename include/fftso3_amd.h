@@ -77,7 +77,10 @@ int fftso3_plan_create_uneven(fftso3_plan_t *plan, int domain, const int *l_max,
 
 int fftso3_plan_destroy(fftso3_plan_t plan);
 
-/* Number of functions a plan processes per pass through the pipeline (tuning knob; 1, 2, 4 or 8). */
+/* Functions per pass through the pipeline.  By default FP32 plans with l_max <= 63 transform each
+ * bandlimit class in one batch-wide pass (the Wigner-d block of every (m,n) group is generated
+ * once and shared by all functions); set_chunk(1|2|4|8) selects the chunked kernels instead.
+ * get_chunk reports the functions per pass of the first bandlimit class. */
 int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk);
 int fftso3_plan_get_chunk(fftso3_plan_t plan, int *chunk);
 

@@ -165,3 +165,43 @@ def test_s2_fp32_b128(fftso3, torch_cuda, oracle):
     for i in range(4):
         assert rel_l2(g[i], oracle.s2_inverse(B, F[i])) < FP32_TOL
         assert rel_l2(c[i], F[i]) < FP32_TOL
+
+
+# split-BF16 contraction on MFMA: BF16X6 (3 pieces, 6 products) is held to the FP32 bar;
+# BF16X3 (2 pieces, 3 products) keeps ~16 significant bits per operand, bar 5e-5.
+SPLIT_TOL = {"bf16x6": FP32_TOL, "bf16x3": 5e-5}
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+def test_so3_split_bf16_b64(fftso3, torch_cuda, oracle, prec):
+    B = 64
+    F = inputs.workload_batch_b64(3)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, F, prec=prec)
+    for i in range(3):
+        assert rel_l2(g[i], oracle.so3_inverse(B, F[i])) < SPLIT_TOL[prec]
+        assert rel_l2(c[i], F[i]) < SPLIT_TOL[prec]
+
+
+@pytest.mark.parametrize("B", [2, 7, 16, 33])
+def test_so3_split_bf16_sizes(fftso3, torch_cuda, oracle, B):
+    """Small and odd bandlimits, 10 functions (a full chunk of 8 and a tail of 2), real and complex."""
+    F = inputs.so3_coefficients(B, 10, 600 + B)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, F, prec="bf16x6")
+    for i in (0, 7, 9):
+        assert rel_l2(g[i], oracle.so3_inverse(B, F[i])) < FP32_TOL
+        assert rel_l2(c[i], F[i]) < FP32_TOL
+    Fr = inputs.so3_coefficients(B, 3, 700 + B, complex_=False)
+    g, c = so3_roundtrip(fftso3, torch_cuda, B, Fr, kind="real", prec="bf16x6")
+    for i in range(3):
+        assert rel_l2(g[i], oracle.so3_real_inverse(B, Fr[i])) < FP32_TOL
+        assert rel_l2(c[i], Fr[i]) < FP32_TOL
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+def test_s2_split_bf16_b128(fftso3, torch_cuda, oracle, prec):
+    B = 128
+    F = inputs.workload_s2(4)
+    g, c = s2_roundtrip(fftso3, torch_cuda, B, F, prec=prec)
+    for i in range(4):
+        assert rel_l2(g[i], oracle.s2_inverse(B, F[i])) < SPLIT_TOL[prec]
+        assert rel_l2(c[i], F[i]) < SPLIT_TOL[prec]
