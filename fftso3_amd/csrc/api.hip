@@ -88,8 +88,10 @@ struct Class {
     int row_tile = 0, col_tile = 0; // two-pass fallback tiles
     bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
     int chunk = 8;                  // functions per pipeline pass for this class
-    DevBuf d_items_mx;              // (g, first function of a block of MX_FPI)
+    DevBuf d_items_mx, d_goff;      // groups (largest first) and their first row in CG
+    DevBuf d_slot;                  // packed coefficient index -> CG row (pair-major)
     int nitems_mx = 0;
+    long cg_rows = 0;               // rows of the group-major coefficient staging array
 };
 
 }  // namespace
@@ -106,8 +108,9 @@ struct fftso3_plan_s {
     bool force_chunked = false;        // set_chunk() selects the chunked FP32 kernels
     DevBuf d_ws;        // group-major spectrum workspace
     DevBuf d_scratch;   // two-pass FFT scratch
-    DevBuf d_fc;        // complex coefficients of the real path
-    size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0;
+    DevBuf d_fc;        // complex coefficients of the real path / group-major staging
+    DevBuf d_tmp;       // packed complex coefficients of the batch-wide real path
+    size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0, tmp_bytes = 0;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -287,11 +290,20 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     c->nitems_fwd_mx = (int)mf.size();
     c->nitems_inv_mx = (int)mi.size();
     if ((st = c->d_items_fwd_mx.upload(mf)) || (st = c->d_items_inv_mx.upload(mi))) return st;
-    std::vector<int2> mx;
-    for (int g = 0; g < ngrp; g++)
-        for (int f = 0; f < (int)c->fns.size(); f += MX_FPI) mx.push_back(make_int2(g, f));
-    c->nitems_mx = (int)mx.size();
-    if ((st = c->d_items_mx.upload(mx))) return st;
+    // batch-wide MFMA path: groups ordered by decreasing row count; CG row offsets per group
+    std::vector<int> mx(ngrp), goff(ngrp);
+    long rows = 0;
+    for (int g = 0; g < ngrp; g++) {
+        int m0 = s2 ? g : (int)std::sqrt((double)g + 0.5);
+        if (!s2) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
+        goff[g] = (int)rows;
+        rows += B - m0;
+        mx[g] = g;
+    }
+    // group g has m0 = floor(sqrt(g)) (SO(3)) or g (S^2): ascending g is descending row count
+    c->cg_rows = rows;
+    c->nitems_mx = ngrp;
+    if ((st = c->d_items_mx.upload(mx)) || (st = c->d_goff.upload(goff))) return st;
     const int CC = c->chunk;
     std::vector<int> ident(CC);
     for (int i = 0; i < CC; i++) ident[i] = i;
@@ -329,7 +341,7 @@ int finalize_plan(fftso3_plan_s *pl)
         }
         byB[B]->fns.push_back(i);
     }
-    size_t ws = 0, scr = 0, fc = 0;
+    size_t ws = 0, scr = 0, fc = 0, tmpb = 0;
     const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
     for (auto *c : pl->classes) {
         if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
@@ -340,10 +352,16 @@ int finalize_plan(fftso3_plan_s *pl)
         if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
         if (pl->kind == FFTSO3_REAL)
             fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
+        if (c->mx) {
+            fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);  // PT
+            if (pl->kind == FFTSO3_REAL) tmpb = std::max(tmpb, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
+        }
     }
     if (ws != pl->ws_bytes) { pl->d_ws.~DevBuf(); new (&pl->d_ws) DevBuf(); if ((st = pl->d_ws.alloc(ws))) return st; }
     if (scr != pl->scratch_bytes) { pl->d_scratch.~DevBuf(); new (&pl->d_scratch) DevBuf(); if ((st = pl->d_scratch.alloc(scr))) return st; }
     if (fc != pl->fc_bytes) { pl->d_fc.~DevBuf(); new (&pl->d_fc) DevBuf(); if ((st = pl->d_fc.alloc(fc))) return st; }
+    if (tmpb != pl->tmp_bytes) { pl->d_tmp.~DevBuf(); new (&pl->d_tmp) DevBuf(); if ((st = pl->d_tmp.alloc(tmpb))) return st; }
+    pl->tmp_bytes = tmpb;
     pl->ws_bytes = ws;
     pl->scratch_bytes = scr;
     pl->fc_bytes = fc;
@@ -405,6 +423,7 @@ ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
     ca.fn = reinterpret_cast<const int *>(c->d_fn.p) + i0;
     ca.grid_off = reinterpret_cast<const long *>(pl->d_grid_off.p);
     ca.coef_off = reinterpret_cast<const long *>(pl->d_coef_off.p);
+    ca.pmajor = 0;
     return ca;
 }
 
@@ -489,16 +508,61 @@ int launch_contract_bf16(Class *c, const ConArgs<float> &a0, bool fwd, hipStream
 }
 
 template <int NP>
-int launch_contract_mx(Class *c, const ConArgs<float> &a0, bool fwd, hipStream_t s)
+int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *in, void *out, hipStream_t s)
 {
-    ConArgs<float> a = a0;
-    a.items = (const int2 *)c->d_items_mx.p;
+    const bool real = pl->kind == FFTSO3_REAL;
+    const bool s2 = NP == 2;
+    MxArgs a;
+    a.ca = ca;
+    a.rt = rec_tables(c);
+    a.items = (const int *)c->d_items_mx.p;
     a.nitems = c->nitems_mx;
-    a.tile = MX_FPI;
     const size_t lds = (size_t)64 * MX_DPITCH * sizeof(float);
-    if (fwd) hipLaunchKernelGGL((contract_fwd_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
-    else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
-    HIP_TRY(hipGetLastError());
+    const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
+    const int tgrid = xcd_grid(((ncp + 63) / 64) * ((ca.nfc + 31) / 32));
+    // the real path stages packed complex coefficients in d_tmp, addressed per class-local function
+    ChunkArgs ct = ca;
+    if (real) {
+        ct.fn = reinterpret_cast<const int *>(c->d_ident.p);
+        ct.coef_off = reinterpret_cast<const long *>(c->d_ws_off.p);
+    }
+    const long nc = ncp;
+    const int tmix_blocks = (int)std::min<long>(4096, (nc * ca.nfc + 255) / 256);
+    if (fwd) {
+        {
+            StageTimer t(pl, s, 1);
+            a.src = pl->d_ws.p;
+            a.dst = pl->d_fc.p;
+            hipLaunchKernelGGL((contract_fwd_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+            HIP_TRY(hipGetLastError());
+        }
+        StageTimer t(pl, s, 2);
+        hipLaunchKernelGGL((coef_transpose<0>), dim3(tgrid), dim3(256), 0, s, ct, ncp, pl->d_fc.p,
+                           real ? pl->d_tmp.p : out);
+        HIP_TRY(hipGetLastError());
+        if (real) {
+            if (s2) hipLaunchKernelGGL((s2_tmix<float, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_tmp.p, out, nc);
+            else hipLaunchKernelGGL((so3_tmix<float, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_tmp.p, out, nc);
+            HIP_TRY(hipGetLastError());
+        }
+    } else {
+        {
+            StageTimer t(pl, s, 3);
+            if (real) {
+                if (s2) hipLaunchKernelGGL((s2_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
+                else hipLaunchKernelGGL((so3_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
+                HIP_TRY(hipGetLastError());
+            }
+            hipLaunchKernelGGL((coef_transpose<1>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
+                               pl->d_fc.p);
+            HIP_TRY(hipGetLastError());
+        }
+        StageTimer t(pl, s, 4);
+        a.src = pl->d_fc.p;
+        a.dst = pl->d_ws.p;
+        hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+        HIP_TRY(hipGetLastError());
+    }
     return FFTSO3_OK;
 }
 
@@ -506,15 +570,6 @@ template <typename T>
 int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
 {
     if constexpr (sizeof(T) == 4) {
-        if (c->mx) {
-            ConArgs<float> a;
-            a.ca = ca;
-            a.rt = rec_tables(c);
-            a.s2 = pl->domain == FFTSO3_DOMAIN_S2;
-            a.src = src;
-            a.dst = dst;
-            return a.s2 ? launch_contract_mx<2>(c, a, fwd, s) : launch_contract_mx<4>(c, a, fwd, s);
-        }
         if (pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6) {
             ConArgs<float> a;
             a.ca = ca;
@@ -591,6 +646,9 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             }
             HIP_TRY(hipGetLastError());
         }
+        if constexpr (sizeof(T) == 4) {
+            if (c->mx) return s2 ? run_mx<2>(pl, c, ca, true, in, out, s) : run_mx<4>(pl, c, ca, true, in, out, s);
+        }
         {
             StageTimer t(pl, s, 1);
             if ((st = contract<T>(pl, c, cc, true, pl->d_ws.p, real ? pl->d_fc.p : out, s))) return st;
@@ -603,14 +661,21 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         }
     } else {
         const void *csrc = in;
-        if (real) {
+        bool contracted = false;
+        if constexpr (sizeof(T) == 4) {
+            if (c->mx) {
+                if ((st = s2 ? run_mx<2>(pl, c, ca, false, in, out, s) : run_mx<4>(pl, c, ca, false, in, out, s))) return st;
+                contracted = true;
+            }
+        }
+        if (real && !contracted) {
             StageTimer t(pl, s, 3);
             if (s2) hipLaunchKernelGGL((s2_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
             else hipLaunchKernelGGL((so3_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
             HIP_TRY(hipGetLastError());
             csrc = pl->d_fc.p;
         }
-        {
+        if (!contracted) {
             StageTimer t(pl, s, 4);
             if ((st = contract<T>(pl, c, cc, false, csrc, pl->d_ws.p, s))) return st;
         }
@@ -850,7 +915,7 @@ long fftso3_plan_coef_elems(fftso3_plan_t plan) { return plan ? plan->coef_total
 long fftso3_plan_workspace_bytes(fftso3_plan_t plan)
 {
     if (!plan) return -1;
-    return (long)(plan->ws_bytes + plan->scratch_bytes + plan->fc_bytes);
+    return (long)(plan->ws_bytes + plan->scratch_bytes + plan->fc_bytes + plan->tmp_bytes);
 }
 
 static int run_dispatch(fftso3_plan_t plan, bool fwd, const void *in, void *out, void *stream)

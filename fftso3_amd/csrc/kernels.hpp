@@ -35,7 +35,15 @@ struct ChunkArgs {
     const int *fn;          // chunk-local index -> global function index (device)
     const long *grid_off;   // global function -> grid offset (elements)
     const long *coef_off;   // global function -> coefficient offset (elements)
+    int pmajor;             // workspace layout: 0 = [g][k][fi][p], 1 = [g][k][p][fi]
 };
+
+// workspace element of (group g, beta index k, pair p, function fi)
+__device__ __forceinline__ long ws_at(const ChunkArgs &ca, int g, int k, int p, int fi, int NP)
+{
+    const long gk = (long)g * ca.N + k;
+    return ca.pmajor ? (gk * NP + p) * ca.C + fi : (gk * ca.C + fi) * NP + p;
+}
 
 // ---------------------------------------------------------------------------------------------
 // SO(3) FFT kernels
@@ -113,11 +121,10 @@ __global__ void __launch_bounds__(1024) so3_fft_fwd_slice(FftArgs<T> a)
         int m0, n0;
         so3_group(g, m0, n0);
         const int um = revl[wrapN(-m0, N)], un = revl[wrapN(-n0, N)], vm = revl[wrapN(m0, N)], vn = revl[wrapN(n0, N)];
-        T2 *o = S + (((long)g * N + k) * ca.C + fi) * 4;
-        o[0] = buf[um * N + un];
-        o[1] = buf[vm * N + vn];
-        o[2] = buf[vn * N + vm];
-        o[3] = buf[un * N + um];
+        S[ws_at(ca, g, k, 0, fi, 4)] = buf[um * N + un];
+        S[ws_at(ca, g, k, 1, fi, 4)] = buf[vm * N + vn];
+        S[ws_at(ca, g, k, 2, fi, 4)] = buf[vn * N + vm];
+        S[ws_at(ca, g, k, 3, fi, 4)] = buf[un * N + um];
     }
 }
 
@@ -146,8 +153,8 @@ __global__ void __launch_bounds__(1024) so3_fft_inv_slice(FftArgs<T> a)
     for (int g = tid; g < B * B; g += nthr) {
         int m0, n0;
         so3_group(g, m0, n0);
-        const T2 *q = G + (((long)g * N + k) * ca.C + fi) * 4;
-        T2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        T2 q0 = G[ws_at(ca, g, k, 0, fi, 4)], q1 = G[ws_at(ca, g, k, 1, fi, 4)];
+        T2 q2 = G[ws_at(ca, g, k, 2, fi, 4)], q3 = G[ws_at(ca, g, k, 3, fi, 4)];
         const int vm = wrapN(m0, N), vn = wrapN(n0, N), um = wrapN(-m0, N), un = wrapN(-n0, N);
         buf[vm * N + vn] = q0;
         if (m0 > 0) {
@@ -247,7 +254,7 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
             if (u != B && v != B) {
                 int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
                 so3_slot(m, n, g, p);
-                val = W[(((long)g * N + k) * ca.C + fi) * 4 + p];
+                val = W[ws_at(ca, g, k, p, fi, 4)];
             }
             return val;
         });
@@ -265,7 +272,7 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
             so3_slot(m, n, g, p);
             // duplicated slots of orbits of size 1 or 2 are left unwritten: the contraction's
             // columns are independent and it never stores results for invalid pairs
-            W[(((long)g * N + k) * ca.C + fi) * 4 + p] = buf[revl[u] * TC + c];
+            W[ws_at(ca, g, k, p, fi, 4)] = buf[revl[u] * TC + c];
         }
     } else {
         for (int e = tid; e < N * TC; e += nthr) {
@@ -311,7 +318,7 @@ __global__ void __launch_bounds__(1024) s2_fft_fwd(FftArgs<T> a)
         int r = e / nq, q = e - r * nq, g = q >> 1, p = q & 1;
         int m = p ? -g : g;
         int k = rb * R + r;
-        S[(((long)g * N + k) * ca.C + fi) * 2 + p] = buf[r * N + revl[wrapN(m, N)]];
+        S[ws_at(ca, g, k, p, fi, 2)] = buf[r * N + revl[wrapN(m, N)]];
     }
 }
 
@@ -339,7 +346,7 @@ __global__ void __launch_bounds__(1024) s2_fft_inv(FftArgs<T> a)
         if (u != B) {
             int m = -(u < B ? u : u - N);   // bin u holds m with -m = u (mod N)
             int g = iabs(m), p = m < 0;
-            val = G[(((long)g * N + k) * ca.C + fi) * 2 + p];
+            val = G[ws_at(ca, g, k, p, fi, 2)];
         }
         return val;
     });
