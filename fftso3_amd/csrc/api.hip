@@ -106,9 +106,10 @@ struct fftso3_plan_s {
     DevBuf d_grid_off, d_coef_off;
     std::vector<Class *> classes;
     bool force_chunked = false;        // set_chunk() selects the chunked FP32 kernels
-    DevBuf d_ws;        // group-major spectrum workspace
+    DevBuf d_ws;        // spectrum workspace (rows of (group, beta index))
     DevBuf d_scratch;   // two-pass FFT scratch
-    DevBuf d_fc;        // complex coefficients of the real path / group-major staging
+    DevBuf d_fc;        // complex coefficients of the real path / transposed coefficients PT
+    DevBuf d_dummy;     // store sink of the batch-wide contraction (lanes with nothing to store)
     DevBuf d_tmp;       // packed complex coefficients of the batch-wide real path
     size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0, tmp_bytes = 0;
     // timing
@@ -322,6 +323,17 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     c->col_tile = rt;
     return FFTSO3_OK;
 }
+// log2 of the run of beta rows a group keeps contiguous in the batch-wide workspace
+static int ws_klo()
+{
+    static const int v = [] {
+        const char *e = getenv("FFTSO3_WS_KLO");
+        const int x = e ? atoi(e) : 0;
+        return x < 0 ? 0 : (x > 7 ? 7 : x);
+    }();
+    return v;
+}
+
 
 int finalize_plan(fftso3_plan_s *pl)
 {
@@ -347,7 +359,8 @@ int finalize_plan(fftso3_plan_s *pl)
         if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
         if ((st = c->d_fn.upload(c->fns))) return st;
         const size_t B = c->B, N = c->N, C = c->chunk;
-        size_t w = s2 ? C * B * N * 2 * pl->esz : C * B * B * N * 4 * pl->esz;
+        const int klo = ws_klo();
+        size_t w = (size_t)ws_rows(c->mx, klo, s2 ? (int)B : (int)(B * B), (int)N) * C * (s2 ? 2 : 4) * pl->esz;
         ws = std::max(ws, w);
         if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
         if (pl->kind == FFTSO3_REAL)
@@ -357,6 +370,10 @@ int finalize_plan(fftso3_plan_s *pl)
             if (pl->kind == FFTSO3_REAL) tmpb = std::max(tmpb, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
         }
     }
+    // ws and PT are read up to 512 bytes past a row start (clamped columns of a partial block)
+    ws += 512;
+    if (fc) fc += 512;
+    if (!pl->d_dummy.p && (st = pl->d_dummy.alloc(256))) return st;
     if (ws != pl->ws_bytes) { pl->d_ws.~DevBuf(); new (&pl->d_ws) DevBuf(); if ((st = pl->d_ws.alloc(ws))) return st; }
     if (scr != pl->scratch_bytes) { pl->d_scratch.~DevBuf(); new (&pl->d_scratch) DevBuf(); if ((st = pl->d_scratch.alloc(scr))) return st; }
     if (fc != pl->fc_bytes) { pl->d_fc.~DevBuf(); new (&pl->d_fc) DevBuf(); if ((st = pl->d_fc.alloc(fc))) return st; }
@@ -423,7 +440,9 @@ ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
     ca.fn = reinterpret_cast<const int *>(c->d_fn.p) + i0;
     ca.grid_off = reinterpret_cast<const long *>(pl->d_grid_off.p);
     ca.coef_off = reinterpret_cast<const long *>(pl->d_coef_off.p);
-    ca.pmajor = 0;
+    ca.kmajor = c->mx ? 1 : 0;
+    ca.klo = ws_klo();
+    ca.ngrp = pl->domain == FFTSO3_DOMAIN_S2 ? c->B : c->B * c->B;
     return ca;
 }
 
@@ -517,7 +536,8 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     a.rt = rec_tables(c);
     a.items = (const int *)c->d_items_mx.p;
     a.nitems = c->nitems_mx;
-    const size_t lds = (size_t)64 * MX_DPITCH * sizeof(float);
+    const size_t lds = MX_LDS_BYTES;
+    a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);
     const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
     const int tgrid = xcd_grid(((ncp + 63) / 64) * ((ca.nfc + 31) / 32));
     // the real path stages packed complex coefficients in d_tmp, addressed per class-local function

@@ -158,18 +158,11 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd(FftArgs<float> a)
         float2 p1 = buf[vm * P128 + vn];     // (-m0, -n0)
         float2 p2 = buf[vn * P128 + vm];     // (-n0, -m0)
         float2 p3 = buf[un * P128 + um];     // (n0, m0)
-        if (ca.pmajor) {
-            float2 *S = reinterpret_cast<float2 *>(a.dst);
-            S[ws_at(ca, g, k, 0, fi, 4)] = p0;
-            S[ws_at(ca, g, k, 1, fi, 4)] = p1;
-            S[ws_at(ca, g, k, 2, fi, 4)] = p2;
-            S[ws_at(ca, g, k, 3, fi, 4)] = p3;
-        } else {  // the 4 pairs are contiguous: two 16-byte stores
-            float4 *S4 = reinterpret_cast<float4 *>(a.dst);
-            const long o = (((long)g * F128 + k) * ca.C + fi) * 2;
-            S4[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
-            S4[o + 1] = make_float4(p2.x, p2.y, p3.x, p3.y);
-        }
+        // the 4 pairs are contiguous: two 16-byte stores
+        float4 *S4 = reinterpret_cast<float4 *>(a.dst);
+        const long o = (ws_row(ca, g, k) * ca.C + fi) * 2;
+        S4[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
+        S4[o + 1] = make_float4(p2.x, p2.y, p3.x, p3.y);
     }
     (void)B;
 }
@@ -192,18 +185,10 @@ __global__ void __launch_bounds__(1024) so3_fft128_inv(FftArgs<float> a)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int g = tid + i * 1024;
-        if (ca.pmajor) {
-            const float2 *G = reinterpret_cast<const float2 *>(a.src);
-            const float2 a0 = G[ws_at(ca, g, k, 0, fi, 4)], a1 = G[ws_at(ca, g, k, 1, fi, 4)];
-            const float2 a2 = G[ws_at(ca, g, k, 2, fi, 4)], a3 = G[ws_at(ca, g, k, 3, fi, 4)];
-            q[i][0] = make_float4(a0.x, a0.y, a1.x, a1.y);
-            q[i][1] = make_float4(a2.x, a2.y, a3.x, a3.y);
-        } else {
-            const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
-            const long o = (((long)g * F128 + k) * ca.C + fi) * 2;
-            q[i][0] = G4[o];
-            q[i][1] = G4[o + 1];
-        }
+        const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
+        const long o = (ws_row(ca, g, k) * ca.C + fi) * 2;
+        q[i][0] = G4[o];
+        q[i][1] = G4[o + 1];
     }
     // Nyquist row and column (m or n = +-64) are zero
     if (tid < F128) {

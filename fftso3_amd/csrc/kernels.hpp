@@ -35,14 +35,31 @@ struct ChunkArgs {
     const int *fn;          // chunk-local index -> global function index (device)
     const long *grid_off;   // global function -> grid offset (elements)
     const long *coef_off;   // global function -> coefficient offset (elements)
-    int pmajor;             // workspace layout: 0 = [g][k][fi][p], 1 = [g][k][p][fi]
+    int kmajor;             // workspace rows: 0 = [g][k] (chunked kernels), 1 = blocked (batch-wide)
+    int klo;                // blocked rows: [k >> klo][g][k & (2^klo - 1)]
+    int ngrp;               // (m,n) groups of the bandlimit: B^2 (SO(3)) or B (S^2)
 };
+
+// Workspace row of (group g, beta index k); a row holds C functions x NP pairs of complex values.
+// The blocked order keeps the rows one FFT slice (one k) touches within 2^klo rows of stride of
+// each other, and a group's rows in runs of 2^klo contiguous rows, so neither the FFT nor the
+// contraction spreads its accesses over many pages.
+__device__ __forceinline__ long ws_row(const ChunkArgs &ca, int g, int k)
+{
+    if (!ca.kmajor) return (long)g * ca.N + k;
+    return ((((long)(k >> ca.klo) * ca.ngrp + g)) << ca.klo) + (k & ((1 << ca.klo) - 1));
+}
+
+// workspace rows of a bandlimit class (the blocked order pads N up to a multiple of 2^klo)
+__host__ __device__ inline long ws_rows(int kmajor, int klo, int ngrp, int N)
+{
+    return kmajor ? (long)(((N + (1 << klo) - 1) >> klo) * ngrp) << klo : (long)ngrp * N;
+}
 
 // workspace element of (group g, beta index k, pair p, function fi)
 __device__ __forceinline__ long ws_at(const ChunkArgs &ca, int g, int k, int p, int fi, int NP)
 {
-    const long gk = (long)g * ca.N + k;
-    return ca.pmajor ? (gk * NP + p) * ca.C + fi : (gk * ca.C + fi) * NP + p;
+    return (ws_row(ca, g, k) * ca.C + fi) * NP + p;
 }
 
 // ---------------------------------------------------------------------------------------------

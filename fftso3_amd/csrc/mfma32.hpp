@@ -1,17 +1,22 @@
 // mfma32.hpp -- batch-wide beta contraction on FP32 MFMA (v_mfma_f32_32x32x2f32), N = 2B <= 128.
 //
-// One workgroup per (m,n) group g for the whole batch: the FP64 recursion writes the group's
-// Wigner-d (or Legendre) block D[l][k] (M = B - m0 <= 64 rows, N <= 128 columns) to LDS once and
-// the four waves stream 32-column tiles of the batch through MFMA, the next tile's operands
-// loading while the current tile multiplies.  Columns are (function, pair, re/im) as in the
-// spectrum workspace [g][k][function][pair].  Coefficients enter and leave through PT[e][function],
-// the transpose of the reference's packed coefficient array (e = packed index of (l,m,n)): the
-// four waves of a workgroup multiply four consecutive column tiles at the same time, so together
-// they read or write whole 128-byte segments of every PT row they touch.
+// One workgroup per (m,n) group g for the whole batch.  The FP64 recursion writes the group's
+// Wigner-d (or Legendre) block D[l][k] (M = B - m0 <= 64 rows, N <= 128 columns) to LDS once; the
+// batch then passes through in blocks of 128 spectrum columns (64/NP functions x NP pairs x re/im,
+// 512 contiguous bytes of every workspace row).  Wave w multiplies the 32 columns of pair
+// p = w % NP of 16 of the block's functions, so each wave's coefficient rows are one pair's: 16
+// functions x 8 bytes = 128 contiguous bytes of a row of PT[e][function], the transpose of the
+// reference's packed coefficient array (e = packed index of (l,m,n)).
+//
+// Forward: the next block's spectrum rows load (16-byte loads, whole 512-byte row segments) into
+// registers while the current block multiplies out of LDS; they are then written to LDS with the
+// columns regrouped by wave.  Inverse: the next block's coefficient rows load while the current
+// block multiplies; results go through LDS so the workspace rows are written as whole 512-byte
+// segments.  Stores never branch (lanes with nothing to store write to a dummy slot), so the
+// compiler's wait counts stay exact and no wait ever covers more than one block's stores.
 //
 // Operand layouts of v_mfma_f32_32x32x2f32 (lane i): A[i%32][i/32], B[i/32][i%32], result
-// D[8(j/4) + 4(i/32) + j%4][i%32] for j < 16.  A lane holds N/2 <= 64 forward and M/2 <= 32
-// inverse operands per tile (B <= 64); offsets within one group's block fit in 32 bits.
+// D[8(j/4) + 4(i/32) + j%4][i%32] for j < 16.
 #pragma once
 
 #include "kernels.hpp"
@@ -19,6 +24,8 @@
 typedef float f32x16m_t __attribute__((ext_vector_type(16)));
 
 #define MX_DPITCH 129        // LDS pitch of D rows (floats): conflict-free column reads
+#define MX_LPITCH 132        // LDS pitch of spectrum-block rows (floats)
+#define MX_LDS_BYTES ((64 * MX_DPITCH + 128 * MX_LPITCH) * 4)
 
 struct MxArgs {
     ChunkArgs ca;            // ca.C = functions of the class (workspace stride)
@@ -27,6 +34,7 @@ struct MxArgs {
     int nitems;
     const void *src;         // forward: spectrum workspace; inverse: PT
     void *dst;               // forward: PT; inverse: spectrum workspace
+    float *dummy;            // 16-byte slot that absorbs the stores of lanes with nothing to store
 };
 
 template <int NP>
@@ -48,24 +56,37 @@ __device__ __forceinline__ bool mx_pair_valid(int m0, int n0, int p)
     return NP == 2 ? (p == 0 || m0 > 0) : so3_pair_valid(m0, n0, p);
 }
 
-// packed index of degree l of pair p of group (m0, n0)
+// (m, n) of pair p of the group, and the packed index of degree l of that pair
 template <int NP>
-__device__ __forceinline__ int mx_packed(int l, int m0, int n0, int p)
-{
-    if (NP == 2) return s2_idx(l, p ? -m0 : m0);
+struct MxPair {
     int m, n;
-    so3_pair(m0, n0, p, m, n);
-    return so3_idx(l, m, n);
+    __device__ __forceinline__ MxPair(int m0, int n0, int p)
+    {
+        if (NP == 2) { m = p ? -m0 : m0; n = 0; } else so3_pair(m0, n0, p, m, n);
+    }
+    __device__ __forceinline__ int packed(int l) const
+    {
+        return NP == 2 ? l * l + m + l : so3_off(l) + (m + l) + (n + l) * (2 * l + 1);
+    }
+};
+
+// LDS column of block column c = (f*NP + p)*2 + ri (f < 64/NP): wave (f/16)*NP + p, lane (f%16)*2 + ri
+template <int NP>
+__device__ __forceinline__ int mx_lcol(int c)
+{
+    const int cc = c >> 1, f = cc / NP, p = cc - f * NP;
+    return (((f >> 4) * NP + p) << 5) + ((f & 15) << 1) + (c & 1);
 }
 
-// Forward: PT(l, pair; fn) = s_p sum_k w_k d^l_{m0 n0}(beta_k) S_k(fn, pair).
+// Forward: PT(packed(l, pair); fn) = s_p sum_k w_k d^l_{m0 n0}(beta_k) S_k(fn, pair).
 template <int NP>
 __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float *D = reinterpret_cast<float *>(smem_raw);     // [64][MX_DPITCH], rows l - m0
+    float *L = D + 64 * MX_DPITCH;                       // [128][MX_LPITCH], spectrum block
     const ChunkArgs &ca = a.ca;
-    const int B = ca.B, N = ca.N, CW = ca.C * NP * 2, NCOL = ca.nfc * NP * 2;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int unit = xcd_unit(a.nitems);
     if (unit >= a.nitems) return;
     const int g = a.items[unit];
@@ -73,161 +94,226 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nrt = (M + 31) / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k = tid; k < N; k += 256) {
-        const double wk = a.rt.w[k];
-        group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
-            D[(l - m0) * MX_DPITCH + k] = (float)(wk * v);
-        });
-        for (int r = M; r < 32 * nrt; r++) D[r * MX_DPITCH + k] = 0.f;
-    }
-    __syncthreads();
-    const float *S = reinterpret_cast<const float *>(a.src) + (size_t)g * N * CW;
-    float *PT = reinterpret_cast<float *>(a.dst);
     const int kh = lane >> 5, cl = lane & 31;
-    const int NCT = (NCOL + 31) / 32;
-    float bA[64], bB[64];
-    // Loads go straight into the operand registers: nothing is applied to a loaded value, so the
-    // loads of a tile stay in flight together.  An invalid column (c >= NCOL) reads column 0 and
-    // rows k >= N read row 0; MFMA columns are independent and invalid ones are never stored, and
-    // tile rows k = 2s + kh with 2s >= N are skipped by the multiply (N is even).
-#define MX_FWD_LOAD(buf, ct_)                                                      \
-    {                                                                              \
-        const int c_ = (ct_) * 32 + cl;                                            \
-        const float *p_ = S + kh * CW + (c_ < NCOL ? c_ : 0);                      \
-        _Pragma("unroll") for (int s = 0; s < 64; s++)                             \
-            buf[s] = p_[(2 * s < N ? s : 0) * 2 * CW];                             \
+    constexpr int FB = 64 / NP;                          // functions per block
+    const int nfb = (ca.nfc + FB - 1) / FB;
+    // load role: rows r0 + 8i (i < 16) of the block, columns 4q..4q+3
+    const int q = tid & 31, r0 = tid >> 5;
+    const int fq = (2 * q) / NP;                         // function (within the block) of the 4 columns
+    const int lc0 = mx_lcol<NP>(4 * q), lc1 = mx_lcol<NP>(4 * q + 2);
+    const float *S = reinterpret_cast<const float *>(a.src);
+    long roff[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, g, min(r0 + 8 * i, N - 1)) * CW + 4 * q;
+    float4 pf[16];
+    auto load_block = [&](int fb) {
+        const long co = fb * FB + fq < ca.nfc ? (long)fb * 128 : 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + roff[i] + co);
+    };
+    load_block(0);
+    const int N8 = (N + 7) & ~7;                         // beta extent multiplied (zero columns past N)
+    for (int k = tid; k < N8; k += 256) {
+        if (k < N) {
+            const double wk = a.rt.w[k];
+            group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
+                D[(l - m0) * MX_DPITCH + k] = (float)(wk * v);
+            });
+            for (int r = M; r < 32 * nrt; r++) D[r * MX_DPITCH + k] = 0.f;
+        } else {
+            for (int r = 0; r < 32 * nrt; r++) D[r * MX_DPITCH + k] = 0.f;
+        }
     }
-#define MX_FWD_TILE(buf, ct_)                                                                    \
-    {                                                                                            \
-        f32x16m_t acc0, acc1;                                                                    \
-        _Pragma("unroll") for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }         \
-        _Pragma("unroll") for (int s = 0; s < 64; s++)                                           \
-        {                                                                                        \
-            if (2 * s < N) {                                                                     \
-                const float a0 = D[cl * MX_DPITCH + 2 * s + kh];                                 \
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, buf[s], acc0, 0, 0, 0);          \
-                if (nrt > 1) {                                                                   \
-                    const float a1 = D[(32 + cl) * MX_DPITCH + 2 * s + kh];                      \
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, buf[s], acc1, 0, 0, 0);      \
-                }                                                                                \
-            }                                                                                    \
-        }                                                                                        \
-        const int c_ = (ct_) * 32 + cl;                                                          \
-        if (c_ < NCOL && mx_pair_valid<NP>(m0, n0, (c_ >> 1) % NP)) {                            \
-            const int p_ = (c_ >> 1) % NP;                                                       \
-            const float sg = (float)mx_pair_sign<NP>(m0, n0, p_);                                \
-            float *o_ = PT + 2 * (c_ / (2 * NP)) + (c_ & 1);                                     \
-            _Pragma("unroll") for (int j = 0; j < 16; j++)                                       \
-            {                                                                                    \
-                const int r = 8 * (j >> 2) + 4 * kh + (j & 3);                                   \
-                if (r < M) o_[2L * mx_packed<NP>(m0 + r, m0, n0, p_) * ca.C] = sg * acc0[j];     \
-                if (nrt > 1 && r + 32 < M)                                                       \
-                    o_[2L * mx_packed<NP>(m0 + r + 32, m0, n0, p_) * ca.C] = sg * acc1[j];       \
-            }                                                                                    \
-        }                                                                                        \
+    // output role of this lane: pair p (uniform over the wave), function fo of the block, re/im cl & 1
+    const int p = __builtin_amdgcn_readfirstlane(wave) % NP, fo = (wave / NP) * 16 + (cl >> 1);
+    const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
+    const bool pvalid = mx_pair_valid<NP>(m0, n0, p);
+    const MxPair<NP> pr(m0, n0, p);
+    float *PT = reinterpret_cast<float *>(a.dst);
+    for (int fb = 0; fb < nfb; fb++) {
+        __syncthreads();                                 // D written / previous block consumed
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            float *row = L + (r0 + 8 * i) * MX_LPITCH;
+            *reinterpret_cast<float2 *>(row + lc0) = make_float2(pf[i].x, pf[i].y);
+            *reinterpret_cast<float2 *>(row + lc1) = make_float2(pf[i].z, pf[i].w);
+        }
+        __syncthreads();
+        if (fb + 1 < nfb) load_block(fb + 1);
+        f32x16m_t acc0, acc1;
+#pragma unroll
+        for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
+        const float *Lw = L + kh * MX_LPITCH + wave * 32 + cl;
+        const float *D0 = D + cl * MX_DPITCH + kh;
+        // rows k >= N of the block hold finite copies of row N-1 and meet zero columns of D
+        if (nrt > 1) {
+            for (int s4 = 0; s4 < N8 / 8; s4++) {
+                float b[4], d0[4], d1[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int s = 4 * s4 + u;
+                    b[u] = Lw[2 * s * MX_LPITCH];
+                    d0[u] = D0[2 * s];
+                    d1[u] = D0[32 * MX_DPITCH + 2 * s];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(d0[u], b[u], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(d1[u], b[u], acc1, 0, 0, 0);
+                }
+            }
+        } else {
+            for (int s4 = 0; s4 < N8 / 8; s4++) {
+                float b[4], d0[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int s = 4 * s4 + u;
+                    b[u] = Lw[2 * s * MX_LPITCH];
+                    d0[u] = D0[2 * s];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(d0[u], b[u], acc0, 0, 0, 0);
+            }
+        }
+        const int fn = fb * FB + fo;
+        const bool ok = pvalid && fn < ca.nfc;
+        float *o = PT + 2 * fn + (cl & 1);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int r = 8 * (j >> 2) + 4 * kh + (j & 3);
+            const long e0 = pr.packed(m0 + min(r, M - 1)), e1 = pr.packed(m0 + min(r + 32, M - 1));
+            float *d0 = ok && r < M ? o + 2 * e0 * C : a.dummy;
+            float *d1 = ok && r + 32 < M ? o + 2 * e1 * C : a.dummy;
+            *d0 = sg * acc0[j];
+            *d1 = sg * acc1[j];
+        }
     }
-    int ct = wave;
-    if (ct < NCT) MX_FWD_LOAD(bA, ct)
-#pragma unroll 1
-    for (; ct < NCT; ct += 8) {
-        if (ct + 4 < NCT) MX_FWD_LOAD(bB, ct + 4)
-        MX_FWD_TILE(bA, ct)
-        if (ct + 4 >= NCT) break;
-        if (ct + 8 < NCT) MX_FWD_LOAD(bA, ct + 8)
-        MX_FWD_TILE(bB, ct + 4)
-    }
-#undef MX_FWD_LOAD
-#undef MX_FWD_TILE
 }
 
-// Inverse: G_k(fn, pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) PT(l, pair; fn) (c_l = 2l+1 for SO(3),
-// 1 for S^2).  Output tile per wave: all k (up to 4 tiles of 32) x 32 columns.
+// Inverse: G_k(fn, pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) PT(packed(l, pair); fn) (c_l = 2l+1
+// for SO(3), 1 for S^2).
 template <int NP>
 __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float *D = reinterpret_cast<float *>(smem_raw);     // [64][MX_DPITCH], rows l - m0
+    float *D = reinterpret_cast<float *>(smem_raw);     // [64][MX_DPITCH], rows l - m0, columns k
+    float *O = D + 64 * MX_DPITCH;                       // [128][MX_LPITCH], output block
     const ChunkArgs &ca = a.ca;
-    const int B = ca.B, N = ca.N, CW = ca.C * NP * 2, NCOL = ca.nfc * NP * 2;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int unit = xcd_unit(a.nitems);
     if (unit >= a.nitems) return;
     const int g = a.items[unit];
     int m0, n0;
     mx_group<NP>(g, m0, n0);
-    const int M = B - m0, M2 = (M + 1) & ~1, nkt = (N + 31) / 32;
+    const int M = B - m0, M4 = (M + 3) & ~3, nkt = (N + 31) / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    constexpr int FB = 64 / NP;
+    const int nfb = (ca.nfc + FB - 1) / FB;
+    // operand role of this lane: pair p, function fo of the block, re/im part cl & 1
+    const int p = __builtin_amdgcn_readfirstlane(wave) % NP, fo = (wave / NP) * 16 + (cl >> 1);
+    const float *PT = reinterpret_cast<const float *>(a.src);
+    const MxPair<NP> pr(m0, n0, p);
+    int eoff[32];
+#pragma unroll
+    for (int s = 0; s < 32; s++) eoff[s] = pr.packed(m0 + min(2 * s + kh, M - 1));
+    float bA[32], bB[32];
+    auto load_block = [&](float *buf, int fb) {
+        const int fn = fb * FB + fo;
+        const float *src = PT + 2 * (fn < ca.nfc ? fn : 0) + (cl & 1);
+#pragma unroll
+        for (int s = 0; s < 32; s++) buf[s] = src[2L * eoff[s] * C];
+    };
+    load_block(bA, 0);
     for (int k = tid; k < 32 * nkt; k += 256) {
         if (k < N) {
             group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * MX_DPITCH + k] = (float)(NP == 2 ? v : (double)(2 * l + 1) * v);
             });
-            if (M2 > M) D[M * MX_DPITCH + k] = 0.f;
+            for (int r = M; r < M4; r++) D[r * MX_DPITCH + k] = 0.f;
         } else {
-            for (int r = 0; r < M2; r++) D[r * MX_DPITCH + k] = 0.f;
+            for (int r = 0; r < M4; r++) D[r * MX_DPITCH + k] = 0.f;
         }
     }
-    __syncthreads();
-    const int kh = lane >> 5, cl = lane & 31;
-    const float *PT = reinterpret_cast<const float *>(a.src);
-    float *G = reinterpret_cast<float *>(a.dst) + (size_t)g * N * CW;
-    const int NCT = (NCOL + 31) / 32;
-    float bA[32], bB[32];
-    // As in the forward kernel, loads go straight into operand registers: an invalid column reads
-    // column 0 and rows l >= M read row m0 (finite coefficients) against zero rows of D (rows up to
-    // M2 are zeroed, later ones skipped).  The pair sign is applied to the result.
-#define MX_INV_LOAD(buf, ct_)                                                        \
-    {                                                                                \
-        const int c_ = (ct_) * 32 + cl;                                              \
-        const int cv_ = c_ < NCOL ? c_ : 0;                                          \
-        const int p_ = (cv_ >> 1) % NP;                                              \
-        const float *i_ = PT + 2 * (cv_ / (2 * NP)) + (cv_ & 1);                     \
-        _Pragma("unroll") for (int s = 0; s < 32; s++)                               \
-        {                                                                            \
-            const int lr = 2 * s + kh;                                               \
-            buf[s] = i_[2L * mx_packed<NP>(m0 + (lr < M ? lr : 0), m0, n0, p_) * ca.C]; \
-        }                                                                            \
+    const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
+    // store role: rows r0 + 8i (i < 16) of the block, columns 4q..4q+3
+    const int q = tid & 31, r0 = tid >> 5;
+    const int fq = (2 * q) / NP;
+    const int lc0 = mx_lcol<NP>(4 * q), lc1 = mx_lcol<NP>(4 * q + 2);
+    float *G = reinterpret_cast<float *>(a.dst);
+    long roff[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) roff[i] = r0 + 8 * i < N ? ws_row(ca, g, r0 + 8 * i) * CW + 4 * q : -1;
+    auto multiply_store = [&](const float *buf, int fb) {
+        f32x16m_t acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int j = 0; j < 16; j++) acc[t][j] = 0.f;
+        // degrees l >= M read finite coefficients (row M-1) against zero rows of D up to M4
+        if (nkt == 4) {
+#pragma unroll
+            for (int s = 0; s < 32; s += 2) {
+                if (4 * (s >> 1) < M4) {
+                    const float *Dr = D + (2 * s + kh) * MX_DPITCH + cl;
+                    float d[2][4];
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int t = 0; t < 4; t++) d[u][t] = Dr[2 * u * MX_DPITCH + 32 * t];
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u][t], buf[s + u], acc[t], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 32; s += 2) {
+                if (4 * (s >> 1) < M4) {
+                    const float *Dr = D + (2 * s + kh) * MX_DPITCH + cl;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (t < nkt) {
+                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(Dr[32 * t], buf[s], acc[t], 0, 0, 0);
+                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(Dr[2 * MX_DPITCH + 32 * t], buf[s + 1],
+                                                                          acc[t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();                                 // previous block's rows stored
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (t < nkt) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int k = t * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+                    O[k * MX_LPITCH + wave * 32 + cl] = sg * acc[t][j];
+                }
+            }
+        }
+        __syncthreads();
+        const bool ok = fb * FB + fq < ca.nfc;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const float *row = O + (r0 + 8 * i) * MX_LPITCH;
+            const float2 u = *reinterpret_cast<const float2 *>(row + lc0);
+            const float2 v = *reinterpret_cast<const float2 *>(row + lc1);
+            float *d = ok && roff[i] >= 0 ? G + roff[i] + (long)fb * 128 : a.dummy;
+            *reinterpret_cast<float4 *>(d) = make_float4(u.x, u.y, v.x, v.y);
+        }
+    };
+    __syncthreads();                                     // D written
+    for (int fb = 0; fb < nfb; fb += 2) {
+        if (fb + 1 < nfb) load_block(bB, fb + 1);
+        multiply_store(bA, fb);
+        if (fb + 1 >= nfb) break;
+        if (fb + 2 < nfb) load_block(bA, fb + 2);
+        multiply_store(bB, fb + 1);
     }
-#define MX_INV_TILE(buf, ct_)                                                                    \
-    {                                                                                            \
-        f32x16m_t acc[4];                                                                        \
-        _Pragma("unroll") for (int t = 0; t < 4; t++)                                            \
-            _Pragma("unroll") for (int j = 0; j < 16; j++) acc[t][j] = 0.f;                      \
-        _Pragma("unroll") for (int s = 0; s < 32; s++)                                           \
-        {                                                                                        \
-            if (2 * s < M2) {                                                                    \
-                const int lr = 2 * s + kh;                                                       \
-                _Pragma("unroll") for (int t = 0; t < 4; t++) if (t < nkt)                       \
-                {                                                                                \
-                    const float av = D[lr * MX_DPITCH + t * 32 + cl];                            \
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, buf[s], acc[t], 0, 0, 0);  \
-                }                                                                                \
-            }                                                                                    \
-        }                                                                                        \
-        const int c_ = (ct_) * 32 + cl;                                                          \
-        if (c_ < NCOL) {                                                                         \
-            const float sg = (float)mx_pair_sign<NP>(m0, n0, (c_ >> 1) % NP);                    \
-            _Pragma("unroll") for (int t = 0; t < 4; t++) if (t < nkt)                           \
-            {                                                                                    \
-                _Pragma("unroll") for (int j = 0; j < 16; j++)                                   \
-                {                                                                                \
-                    const int k = t * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);                      \
-                    if (k < N) G[k * CW + c_] = sg * acc[t][j];                                  \
-                }                                                                                \
-            }                                                                                    \
-        }                                                                                        \
-    }
-    int ct = wave;
-    if (ct < NCT) MX_INV_LOAD(bA, ct)
-#pragma unroll 1
-    for (; ct < NCT; ct += 8) {
-        if (ct + 4 < NCT) MX_INV_LOAD(bB, ct + 4)
-        MX_INV_TILE(bA, ct)
-        if (ct + 4 >= NCT) break;
-        if (ct + 8 < NCT) MX_INV_LOAD(bA, ct + 8)
-        MX_INV_TILE(bB, ct + 4)
-    }
-#undef MX_INV_LOAD
-#undef MX_INV_TILE
 }
 
 // ---------------------------------------------------------------------------------------------
