@@ -415,7 +415,7 @@ int g_num_cus = 0;
 
 // persistent grid: every CU filled to its occupancy for this kernel and LDS size, multiple of 8
 template <typename K>
-int persistent_grid(K kernel, size_t lds, int nunits)
+int persistent_grid(K kernel, size_t lds, int nunits, int threads = kFftThreads)
 {
     if (g_num_cus == 0) {
         int dev = 0;
@@ -423,7 +423,7 @@ int persistent_grid(K kernel, size_t lds, int nunits)
         if (hipDeviceGetAttribute(&g_num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) g_num_cus = 256;
     }
     int per = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kFftThreads, lds) != hipSuccess || per < 1) per = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds) != hipSuccess || per < 1) per = 1;
     int grid = g_num_cus * per;
     grid = std::min(grid, xcd_grid(nunits));
     return std::max(XCD_COUNT, (grid / XCD_COUNT) * XCD_COUNT);
@@ -553,7 +553,8 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             StageTimer t(pl, s, 1);
             a.src = pl->d_ws.p;
             a.dst = pl->d_fc.p;
-            hipLaunchKernelGGL((contract_fwd_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+            if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+            else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
             HIP_TRY(hipGetLastError());
         }
         StageTimer t(pl, s, 2);
@@ -797,8 +798,10 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv<double, 2>));
     ALLOW_LDS((contract_inv<double, 4>));
     ALLOW_LDS((contract_inv<double, 8>));
-    ALLOW_LDS((contract_fwd_mx32<4>));
-    ALLOW_LDS((contract_fwd_mx32<2>));
+    ALLOW_LDS((contract_fwd_mx32<4, true>));
+    ALLOW_LDS((contract_fwd_mx32<2, true>));
+    ALLOW_LDS((contract_fwd_mx32<4, false>));
+    ALLOW_LDS((contract_fwd_mx32<2, false>));
     ALLOW_LDS((contract_inv_mx32<4>));
     ALLOW_LDS((contract_inv_mx32<2>));
     ALLOW_LDS((so3_fft128_fwd<true>));

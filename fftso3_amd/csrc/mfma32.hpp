@@ -79,7 +79,9 @@ __device__ __forceinline__ int mx_lcol(int c)
 }
 
 // Forward: PT(packed(l, pair); fn) = s_p sum_k w_k d^l_{m0 n0}(beta_k) S_k(fn, pair).
-template <int NP>
+// FULL: N == 128; the lane's operands of D (64 per row tile) then stay in registers for the whole
+// batch and the multiply is unrolled over all 64 steps, with two independent accumulators.
+template <int NP, bool FULL>
 __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -130,6 +132,17 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     const bool pvalid = mx_pair_valid<NP>(m0, n0, p);
     const MxPair<NP> pr(m0, n0, p);
     float *PT = reinterpret_cast<float *>(a.dst);
+    const float *Lw = L + kh * MX_LPITCH + wave * 32 + cl;
+    const float *D0 = D + cl * MX_DPITCH + kh;
+    float dA[FULL ? 64 : 1], dB[FULL ? 64 : 1];
+    if constexpr (FULL) {
+        __syncthreads();                                 // D written
+#pragma unroll
+        for (int s = 0; s < 64; s++) {
+            dA[s] = D0[2 * s];
+            dB[s] = nrt > 1 ? D0[32 * MX_DPITCH + 2 * s] : 0.f;
+        }
+    }
     for (int fb = 0; fb < nfb; fb++) {
         __syncthreads();                                 // D written / previous block consumed
 #pragma unroll
@@ -143,10 +156,32 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
         f32x16m_t acc0, acc1;
 #pragma unroll
         for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
-        const float *Lw = L + kh * MX_LPITCH + wave * 32 + cl;
-        const float *D0 = D + cl * MX_DPITCH + kh;
-        // rows k >= N of the block hold finite copies of row N-1 and meet zero columns of D
-        if (nrt > 1) {
+        if constexpr (FULL) {
+            if (nrt > 1) {
+#pragma unroll
+                for (int s0 = 0; s0 < 64; s0 += 8) {
+                    float b[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) b[u] = Lw[2 * (s0 + u) * MX_LPITCH];
+                    __builtin_amdgcn_sched_barrier(0);   // keep the 8 reads ahead of the MFMAs
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(dA[s0 + u], b[u], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(dB[s0 + u], b[u], acc1, 0, 0, 0);
+                    }
+                }
+            } else {
+                // even and odd steps in separate accumulators (independent MFMA chains), summed
+#pragma unroll
+                for (int s = 0; s < 64; s += 2) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(dA[s], Lw[2 * s * MX_LPITCH], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(dA[s + 1], Lw[(2 * s + 2) * MX_LPITCH], acc1, 0, 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) { acc0[j] += acc1[j]; acc1[j] = 0.f; }
+            }
+        } else if (nrt > 1) {
+            // rows k >= N of the block hold finite copies of row N-1 and meet zero columns of D
             for (int s4 = 0; s4 < N8 / 8; s4++) {
                 float b[4], d0[4], d1[4];
 #pragma unroll
