@@ -8,7 +8,9 @@ LIB = os.path.join(HERE, "libfftso3_amd.so")
 FDCL_LIB = os.path.join(HERE, "libfftso3_fdcl.so")
 
 HIP_SOURCES = [os.path.join(HERE, "csrc", "api.hip")]
-HIP_DEPS = [os.path.join(HERE, "csrc", f) for f in ("common.hpp", "fft.hpp", "kernels.hpp", "mfma.hpp")]
+HIP_DEPS = [os.path.join(HERE, "csrc", f)
+            for f in ("common.hpp", "fft.hpp", "fft128.hpp", "kernels.hpp", "mfma.hpp", "mfma32.hpp")]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -42,13 +44,32 @@ def build_fdcl(verbose=False):
     """Compile the C++ class layer (fdcl:: drop-in) into fftso3_amd/libfftso3_fdcl.so."""
     src_dir = os.path.join(HERE, "cpp", "src")
     srcs = sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir) if f.endswith(".cpp"))
-    cmd = ["g++", "-O2", "-std=c++14", "-fPIC", "-shared", "-fopenmp",
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
            "-I", os.path.join(HERE, "cpp", "include"), "-I", os.path.join(ROOT, "include"),
-           "-o", FDCL_LIB] + srcs + ["-L", HERE, "-lfftso3_amd", "-Wl,-rpath,$ORIGIN"]
+           "-I", os.path.join(ROCM, "include"),
+           "-o", FDCL_LIB] + srcs + ["-L", HERE, "-lfftso3_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+                                     "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     out = _run(cmd)
     if verbose and out.strip():
         print(out)
     return FDCL_LIB
+
+
+FDCL_TEST = os.path.join(ROOT, "tests", "cpp", "test_fdcl")
+
+
+def build_fdcl_test(verbose=False):
+    """Compile the class layer's check program tests/cpp/test_fdcl (links libfftso3_fdcl.so)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(HERE, "cpp", "include"), "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(ROCM, "include"), "-o", FDCL_TEST,
+           os.path.join(ROOT, "tests", "cpp", "test_fdcl.cpp"),
+           "-L", HERE, "-lfftso3_fdcl", "-lfftso3_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath," + HERE, "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    out = _run(cmd)
+    if verbose and out.strip():
+        print(out)
+    return FDCL_TEST
 
 
 def build_all(verbose=False):
@@ -56,3 +77,4 @@ def build_all(verbose=False):
     if os.path.isdir(os.path.join(HERE, "cpp", "src")) and any(
             f.endswith(".cpp") for f in os.listdir(os.path.join(HERE, "cpp", "src"))):
         build_fdcl(verbose)
+        build_fdcl_test(verbose)

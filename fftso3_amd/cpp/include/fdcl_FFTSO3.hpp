@@ -1,0 +1,164 @@
+// fdcl_FFTSO3.hpp -- fdcl::FFTSO3_complex / fdcl::FFTSO3_real on MI355X.
+//
+// Drop-in for the reference's classes (include/fdcl_FFTSO3.hpp:27-269): same names, members and
+// overloads for construction, Wigner functions, forward / inverse transforms and the self
+// checks.  The forward transform runs on the GPU through the C ABI of libfftso3_amd.so
+// (include/fftso3_amd.h); pointwise evaluations (inverse_transform at one rotation, wigner_d,
+// wigner_D, real_harmonics) stay on the host in FP64, as they are O(B^3) per call.
+//
+// Additions over the reference (it has no grid or batch entry points):
+//   forward_transform(grid)            grid [k][j1][j2] of (2B)^3 samples -> coefficients
+//   inverse_transform(F)               coefficients -> the whole (2B)^3 grid
+//   forward_transform_batch / inverse_transform_batch
+//   set_precision(FFTSO3_FP64 | FFTSO3_FP32 | FFTSO3_BF16X3 | FFTSO3_BF16X6)
+//
+// Differences forced by the environment: there is no Eigen, so rotations are fdcl::Matrix3d
+// (a 3x3 row-major value type with R(i,j)) and F[l] is a column-major view (fdcl::BlockView).
+// Errors of the device path throw std::runtime_error carrying the library's message; the
+// reference only asserts.
+#ifndef FDCL_FFTSO3_HPP
+#define FDCL_FFTSO3_HPP
+
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "fdcl_FFTSO3_matrix.hpp"
+#include "fftso3_amd.h"
+
+namespace fdcl
+{
+
+// 3x3 rotation matrix, row-major: R(i,j) is row i, column j.
+struct Matrix3d {
+    double a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    double &operator()(int i, int j) { return a[3 * i + j]; }
+    double operator()(int i, int j) const { return a[3 * i + j]; }
+};
+
+// 3-2-3 Euler angles <-> rotation matrix (src/misc_matrix_func.cpp:144-178)
+std::vector<double> R2Euler323(const Matrix3d &R);
+Matrix3d Euler3232R(double a, double b, double g);
+
+namespace detail
+{
+// A device plan of the C ABI with its own device grid / coefficient buffers, created on first
+// use and kept while the (domain, l_max, n_fn, kind, precision) it was made for is unchanged.
+class DevicePlan
+{
+public:
+    ~DevicePlan();
+    // grid / coefficient arrays of n_fn functions, host side in double or complex<double>
+    void forward(int domain, int l_max, int n_fn, int kind, int precision, const void *grid, void *coef);
+    void inverse(int domain, int l_max, int n_fn, int kind, int precision, const void *coef, void *grid);
+
+private:
+    void prepare(int domain, int l_max, int n_fn, int kind, int precision);
+    fftso3_plan_t plan_ = nullptr;
+    int key_[5] = {-1, -1, -1, -1, -1};
+    void *d_grid_ = nullptr, *d_coef_ = nullptr;
+    long grid_elems_ = 0, coef_elems_ = 0;
+    std::vector<char> stage_;  // host staging for the single-precision plans
+};
+}  // namespace detail
+
+class FFTSO3_complex
+{
+public:
+    int l_max = 0;
+    bool check_verbose = false;
+
+    FFTSO3_complex() {}
+    explicit FFTSO3_complex(int l_max);
+    virtual ~FFTSO3_complex() {}
+    void init(int l_max);
+
+    // arithmetic of the device transforms (FFTSO3_FP64 by default: the exact drop-in)
+    void set_precision(int precision) { precision_ = precision; }
+    int precision() const { return precision_; }
+
+    // Wigner small-d d^l_{mn}(beta), l <= l_max (src/fdcl_FFTSO3.cpp:127-194)
+    FFTSO3_matrix_real wigner_d(double beta);
+    // Wigner D^l_{mn} = exp(-i m alpha) d^l_{mn}(beta) exp(-i n gamma) (src/fdcl_FFTSO3.cpp:201-224)
+    FFTSO3_matrix_complex wigner_D(const Matrix3d &R);
+    FFTSO3_matrix_complex wigner_D(double alpha, double beta, double gamma);
+
+    // forward transform of f sampled on the grid (src/fdcl_FFTSO3.cpp:654-763); the callback is
+    // called concurrently from OpenMP threads, as in the reference
+    FFTSO3_matrix_complex forward_transform(std::function<complex<double>(double, double, double)> func);
+    FFTSO3_matrix_complex forward_transform(std::function<complex<double>(Matrix3d)> func);
+    // grid [k][j1][j2] = f(alpha_j1, beta_k, gamma_j2), (2B)^3 values
+    FFTSO3_matrix_complex forward_transform(const std::vector<complex<double>> &grid);
+    std::vector<FFTSO3_matrix_complex> forward_transform_batch(const std::vector<complex<double>> &grids, int n_fn);
+
+    // pointwise inverse; like the reference, re-initialises the object to F.l_max
+    // (src/fdcl_FFTSO3.cpp:498-531)
+    complex<double> inverse_transform(FFTSO3_matrix_complex F, double alpha, double beta, double gamma);
+    complex<double> inverse_transform(FFTSO3_matrix_complex F, const Matrix3d &R);
+    // inverse on the whole grid [k][j1][j2] (on the GPU)
+    std::vector<complex<double>> inverse_transform(const FFTSO3_matrix_complex &F);
+    std::vector<complex<double>> inverse_transform_batch(const std::vector<FFTSO3_matrix_complex> &F);
+
+    // character chi_l(theta) = sin((2l+1) theta / 2) / sin(theta / 2) (src/fdcl_FFTSO3.cpp:404-415)
+    std::vector<double> character(double theta);
+
+    // self checks of the reference (src/fdcl_FFTSO3.cpp:258-415,532-545,1483-1571); each returns an
+    // error measure that is ~0 when the implementation is right
+    void check_all();
+    double check_weight();
+    double check_wigner_d();
+    double check_transform();
+
+    // grid nodes
+    double alpha_j(int j) const;
+    double beta_k(int k) const;
+    double gamma_j(int j) const;
+    std::vector<double> weight;  // quadrature weights w_k, k < 2B (src/fdcl_FFTSO3.cpp:236-256)
+
+protected:
+    int B = 1;
+    int precision_ = FFTSO3_FP64;
+    std::vector<double> compute_weight();
+    FFTSO3_matrix_real wigner_d(double beta, int L);
+    FFTSO3_matrix_complex wigner_D(double alpha, double beta, double gamma, int L);
+    std::shared_ptr<detail::DevicePlan> dev_ = std::make_shared<detail::DevicePlan>();
+};
+
+class FFTSO3_real : public FFTSO3_complex
+{
+public:
+    FFTSO3_real() {}
+    explicit FFTSO3_real(int l_max);
+    void init(int l_max);
+
+    // real harmonics U^l = conj(T^l) D^l (T^l)^T (src/fdcl_FFTSO3.cpp:1015-1105)
+    FFTSO3_matrix_real real_harmonics(double alpha, double beta, double gamma);
+    FFTSO3_matrix_real real_harmonics(const Matrix3d &R);
+
+    // real forward transform (src/fdcl_FFTSO3.cpp:774-850, 1169-1181)
+    FFTSO3_matrix_real forward_transform(std::function<double(double, double, double)> func);
+    FFTSO3_matrix_real forward_transform(std::function<double(Matrix3d)> func);
+    FFTSO3_matrix_real forward_transform(const std::vector<double> &grid);
+    std::vector<FFTSO3_matrix_real> forward_transform_batch(const std::vector<double> &grids, int n_fn);
+
+    // pointwise real inverse f = sum (2l+1) F_r U (src/fdcl_FFTSO3.cpp:1141-1167); uses l_max
+    double inverse_transform(FFTSO3_matrix_real F, double alpha, double beta, double gamma);
+    double inverse_transform(FFTSO3_matrix_real F, const Matrix3d &R);
+    std::vector<double> inverse_transform(const FFTSO3_matrix_real &F);
+    std::vector<double> inverse_transform_batch(const std::vector<FFTSO3_matrix_real> &F);
+
+    // T^l of src/fdcl_FFTSO3.cpp:1107-1139
+    FFTSO3_matrix_complex matrix2rsph(int L);
+
+    void check_all();
+    double check_transform();
+    double check_real_harmonics();
+
+protected:
+    FFTSO3_matrix_real real_harmonics(double alpha, double beta, double gamma, int L);
+};
+
+}  // namespace fdcl
+
+#endif

@@ -1,0 +1,471 @@
+// fdcl_FFTSO3.cpp -- host side of fdcl::FFTSO3_complex / fdcl::FFTSO3_real: FP64 Wigner functions,
+// quadrature weights, pointwise inverses and self checks; transforms go to the GPU.
+#include <omp.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <iostream>
+
+#include "fdcl_FFTSO3.hpp"
+
+namespace fdcl
+{
+
+// ---------------------------------------------------------------------------------------------
+// Wigner small-d for every l <= L at one beta.
+//
+// The reference seeds each (m, n) column with closed forms and runs recursion (64)
+// (src/fdcl_FFTSO3.cpp:127-194).  Here every group (m0, n0), m0 >= |n0|, starts at l = m0 from
+//   d^{m0}_{m0 n0}(beta) = sqrt((2 m0)! / ((m0+n0)! (m0-n0)!)) cos(beta/2)^(m0+n0) (-sin(beta/2))^(m0-n0)
+// evaluated in logarithms (no overflow or underflow of the factorials for large l), then the
+// three-term recursion in l
+//   (l-1) sqrt((l^2-m^2)(l^2-n^2)) d^l = (2l-1)(l(l-1) cos beta - m n) d^{l-1}
+//                                        - l sqrt(((l-1)^2-m^2)((l-1)^2-n^2)) d^{l-2},
+// and the other three members of the group follow from the symmetries of src/fdcl_FFTSO3.cpp:174-190:
+//   d(l,m,n) = (-1)^(m+n) d(l,-m,-n) = d(l,-n,-m) = (-1)^(m+n) d(l,n,m).
+// ---------------------------------------------------------------------------------------------
+static void wigner_d_all(double beta, int L, FFTSO3_matrix_real &d)
+{
+    d.init(L);
+    const double cb = std::cos(beta), ch = std::cos(0.5 * beta), sh = std::sin(0.5 * beta);
+    const double lc = std::log(std::fabs(ch)), ls = std::log(std::fabs(sh));
+    for (int m0 = 0; m0 <= L; m0++) {
+        for (int n0 = -m0; n0 <= m0; n0++) {
+            const int a = m0 + n0, b = m0 - n0;
+            // seed magnitude and sign; x^0 = 1 also at x = 0
+            double lg = 0.5 * (std::lgamma(2.0 * m0 + 1) - std::lgamma(a + 1.0) - std::lgamma(b + 1.0));
+            if (a) lg += a * lc;
+            if (b) lg += b * ls;
+            double v = (a && ch == 0.0) || (b && sh == 0.0) ? 0.0 : std::exp(lg);
+            if ((a & 1) && ch < 0) v = -v;
+            if ((b & 1) && sh > 0) v = -v;  // (-sin(beta/2))^b
+            double vm1 = 0.0;
+            const double mn = (double)m0 * n0;
+            const double sgn = ((m0 + n0) & 1) ? -1.0 : 1.0;
+            for (int l = m0; l <= L; l++) {
+                if (l > m0) {
+                    double nv;
+                    if (l == 1) {
+                        nv = cb;  // d^1_00
+                    } else {
+                        const double l1 = l - 1.0;
+                        const double c1 = (2.0 * l - 1.0) * (l * l1 * cb - mn);
+                        const double c2 = l * std::sqrt((l1 * l1 - (double)m0 * m0) * (l1 * l1 - (double)n0 * n0));
+                        const double den = l1 * std::sqrt(((double)l * l - (double)m0 * m0) * ((double)l * l - (double)n0 * n0));
+                        nv = (c1 * v - c2 * vm1) / den;
+                    }
+                    vm1 = v;
+                    v = nv;
+                }
+                d(l, m0, n0) = v;
+                d(l, -m0, -n0) = sgn * v;
+                d(l, -n0, -m0) = v;
+                d(l, n0, m0) = sgn * v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FFTSO3_complex
+// ---------------------------------------------------------------------------------------------
+FFTSO3_complex::FFTSO3_complex(int l_max_) { init(l_max_); }
+
+void FFTSO3_complex::init(int l_max_)
+{
+    if (l_max_ == l_max && (int)weight.size() == 2 * (l_max_ + 1)) return;
+    l_max = l_max_;
+    B = l_max + 1;
+    weight.assign(2 * B, 0.0);
+    compute_weight();
+}
+
+double FFTSO3_complex::alpha_j(int j) const { return (double)j * M_PI / (double)B; }
+double FFTSO3_complex::gamma_j(int j) const { return (double)j * M_PI / (double)B; }
+double FFTSO3_complex::beta_k(int k) const { return (double)(2 * k + 1) * M_PI / (4.0 * B); }
+
+// w_k = sin(beta_k) / (4 B^3) sum_{i<B} sin((2k+1)(2i+1) pi / 4B) / (2i+1) (src/fdcl_FFTSO3.cpp:236-256)
+std::vector<double> FFTSO3_complex::compute_weight()
+{
+    const double f = M_PI / (4.0 * B);
+    weight.assign(2 * B, 0.0);
+    for (int k = 0; k < 2 * B; k++) {
+        double s = 0.0;
+        for (int i = 0; i < B; i++) s += std::sin((double)((2 * k + 1) * (2 * i + 1)) * f) / (2 * i + 1);
+        weight[k] = s * std::sin((2 * k + 1) * f) / (4.0 * B * B * B);
+    }
+    return weight;
+}
+
+FFTSO3_matrix_real FFTSO3_complex::wigner_d(double beta, int L)
+{
+    FFTSO3_matrix_real d;
+    wigner_d_all(beta, L, d);
+    return d;
+}
+
+FFTSO3_matrix_real FFTSO3_complex::wigner_d(double beta) { return wigner_d(beta, l_max); }
+
+FFTSO3_matrix_complex FFTSO3_complex::wigner_D(double alpha, double beta, double gamma, int L)
+{
+    FFTSO3_matrix_real d = wigner_d(beta, L);
+    FFTSO3_matrix_complex D(L);
+    for (int l = 0; l <= L; l++)
+        for (int m = -l; m <= l; m++)
+            for (int n = -l; n <= l; n++)
+                D(l, m, n) = std::exp(-I * ((double)m * alpha + (double)n * gamma)) * d(l, m, n);
+    return D;
+}
+
+FFTSO3_matrix_complex FFTSO3_complex::wigner_D(double alpha, double beta, double gamma)
+{
+    return wigner_D(alpha, beta, gamma, l_max);
+}
+
+FFTSO3_matrix_complex FFTSO3_complex::wigner_D(const Matrix3d &R)
+{
+    const std::vector<double> abg = R2Euler323(R);
+    return wigner_D(abg[0], abg[1], abg[2], l_max);
+}
+
+std::vector<double> FFTSO3_complex::character(double theta)
+{
+    std::vector<double> chi(l_max + 1);
+    for (int l = 0; l <= l_max; l++) chi[l] = std::sin((2.0 * l + 1) / 2. * theta) / std::sin(theta / 2.);
+    return chi;
+}
+
+FFTSO3_matrix_complex FFTSO3_complex::forward_transform(std::function<complex<double>(double, double, double)> func)
+{
+    const int N = 2 * B;
+    std::vector<complex<double>> grid((size_t)N * N * N);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < N; k++)
+        for (int j1 = 0; j1 < N; j1++)
+            for (int j2 = 0; j2 < N; j2++)
+                grid[((size_t)k * N + j1) * N + j2] = func(alpha_j(j1), beta_k(k), gamma_j(j2));
+    return forward_transform(grid);
+}
+
+FFTSO3_matrix_complex FFTSO3_complex::forward_transform(std::function<complex<double>(Matrix3d)> func)
+{
+    return forward_transform([&](double a, double b, double g) { return func(Euler3232R(a, b, g)); });
+}
+
+FFTSO3_matrix_complex FFTSO3_complex::forward_transform(const std::vector<complex<double>> &grid)
+{
+    return forward_transform_batch(grid, 1)[0];
+}
+
+std::vector<FFTSO3_matrix_complex> FFTSO3_complex::forward_transform_batch(const std::vector<complex<double>> &grids,
+                                                                           int n_fn)
+{
+    const long N = 2 * B, ng = N * N * N;
+    if (n_fn < 1 || (long)grids.size() != ng * n_fn)
+        throw std::invalid_argument("forward_transform_batch: expected n_fn grids of (2B)^3 values");
+    std::vector<complex<double>> coef(FFTSO3_matrix_complex::size_for(l_max) * n_fn);
+    dev_->forward(FFTSO3_DOMAIN_SO3, l_max, n_fn, FFTSO3_COMPLEX, precision_, grids.data(), coef.data());
+    std::vector<FFTSO3_matrix_complex> out(n_fn);
+    const long nc = FFTSO3_matrix_complex::size_for(l_max);
+    for (int i = 0; i < n_fn; i++) {
+        out[i].l_max = l_max;
+        out[i].v.assign(coef.begin() + i * nc, coef.begin() + (i + 1) * nc);
+    }
+    return out;
+}
+
+complex<double> FFTSO3_complex::inverse_transform(FFTSO3_matrix_complex F, double alpha, double beta, double gamma)
+{
+    init(F.l_max);
+    const FFTSO3_matrix_real d = wigner_d(beta, l_max);
+    // f = sum_{m,n} exp(-i m alpha) [sum_l (2l+1) F d] exp(-i n gamma)
+    complex<double> f = 0.0;
+    for (int m = -l_max; m <= l_max; m++) {
+        const complex<double> em = std::exp(-I * ((double)m * alpha));
+        for (int n = -l_max; n <= l_max; n++) {
+            complex<double> t = 0.0;
+            for (int l = std::max(std::abs(m), std::abs(n)); l <= l_max; l++)
+                t += (double)(2 * l + 1) * F(l, m, n) * d(l, m, n);
+            f += em * t * std::exp(-I * ((double)n * gamma));
+        }
+    }
+    return f;
+}
+
+complex<double> FFTSO3_complex::inverse_transform(FFTSO3_matrix_complex F, const Matrix3d &R)
+{
+    const std::vector<double> abg = R2Euler323(R);
+    return inverse_transform(std::move(F), abg[0], abg[1], abg[2]);
+}
+
+std::vector<complex<double>> FFTSO3_complex::inverse_transform(const FFTSO3_matrix_complex &F)
+{
+    return inverse_transform_batch(std::vector<FFTSO3_matrix_complex>{F});
+}
+
+std::vector<complex<double>> FFTSO3_complex::inverse_transform_batch(const std::vector<FFTSO3_matrix_complex> &F)
+{
+    if (F.empty()) return {};
+    init(F[0].l_max);
+    const long nc = FFTSO3_matrix_complex::size_for(l_max), N = 2 * B;
+    std::vector<complex<double>> coef(nc * F.size());
+    for (size_t i = 0; i < F.size(); i++) {
+        if (F[i].l_max != l_max) throw std::invalid_argument("inverse_transform_batch: mixed l_max");
+        std::copy(F[i].v.begin(), F[i].v.end(), coef.begin() + i * nc);
+    }
+    std::vector<complex<double>> grid(N * N * N * F.size());
+    dev_->inverse(FFTSO3_DOMAIN_SO3, l_max, (int)F.size(), FFTSO3_COMPLEX, precision_, coef.data(), grid.data());
+    return grid;
+}
+
+// sum_k w_k d^l_00(beta_k) 4B^2 = delta_{l0} for l < 2B (src/fdcl_FFTSO3.cpp:258-288)
+double FFTSO3_complex::check_weight()
+{
+    compute_weight();
+    std::vector<double> sum(2 * B, 0.0);
+    for (int k = 0; k < 2 * B; k++) {
+        const FFTSO3_matrix_real d = wigner_d(beta_k(k), 2 * B - 1);
+        for (int l = 0; l < 2 * B; l++) sum[l] += d(l, 0, 0) * weight[k];
+    }
+    double err = std::fabs(sum[0] * 4.0 * B * B - 1.0);
+    for (int l = 1; l < 2 * B; l++) err += std::fabs(sum[l] * 4.0 * B * B);
+    if (check_verbose)
+        for (int l = 0; l < 2 * B; l++) std::cout << "l=" << l << ": " << sum[l] * 4.0 * B * B << std::endl;
+    std::cout << "fdcl::FFTSO3_complex::check_weight: error = " << err << std::endl;
+    return err;
+}
+
+// Orthogonality d^l (d^l)^T = I at a random beta, and the closed forms of degree 1
+// (src/fdcl_FFTSO3.cpp:16-120, 320-392)
+double FFTSO3_complex::check_wigner_d()
+{
+    const int L = std::max(l_max, 1);
+    const double beta = (double)std::rand() / RAND_MAX * M_PI;
+    const FFTSO3_matrix_real d = wigner_d(beta, L);
+    double e_orth = 0.0;
+    for (int l = 0; l <= L; l++) {
+        const auto b = d[l];
+        for (int i = 0; i < 2 * l + 1; i++)
+            for (int j = 0; j < 2 * l + 1; j++) {
+                double s = 0.0;
+                for (int q = 0; q < 2 * l + 1; q++) s += b(q, i) * b(q, j);
+                e_orth = std::max(e_orth, std::fabs(s - (i == j ? 1.0 : 0.0)));
+            }
+    }
+    const double c = std::cos(beta), s = std::sin(beta), r2 = std::sqrt(2.0);
+    const double d1[3][3] = {{(1 + c) / 2, s / r2, (1 - c) / 2},     // m = -1
+                             {-s / r2, c, s / r2},                  // m = 0
+                             {(1 - c) / 2, -s / r2, (1 + c) / 2}};  // m = 1
+    double e_exp = std::fabs(d(0, 0, 0) - 1.0);
+    for (int m = -1; m <= 1; m++)
+        for (int n = -1; n <= 1; n++) e_exp = std::max(e_exp, std::fabs(d(1, m, n) - d1[m + 1][n + 1]));
+    const double err = std::max(e_orth, e_exp);
+    if (check_verbose) std::cout << "orthogonality " << e_orth << ", closed form l=1 " << e_exp << std::endl;
+    std::cout << "fdcl::FFTSO3_complex::check_wigner_d: error = " << err << std::endl;
+    return err;
+}
+
+// f = inverse_transform(F) for random F, then forward_transform(f) must return F
+// (src/fdcl_FFTSO3.cpp:532-545)
+double FFTSO3_complex::check_transform()
+{
+    FFTSO3_matrix_complex F(l_max);
+    F.setRandom();
+    const FFTSO3_matrix_complex Fc = F;
+    FFTSO3_matrix_complex F2 = forward_transform(
+        [&](double a, double b, double g) { return inverse_transform(Fc, a, b, g); });
+    const double err = (F2 - F).norm();
+    std::cout << "fdcl::FFTSO3_complex::check_transform: error = " << err << std::endl;
+    return err;
+}
+
+void FFTSO3_complex::check_all()
+{
+    check_weight();
+    check_wigner_d();
+    check_transform();
+}
+
+// ---------------------------------------------------------------------------------------------
+// FFTSO3_real
+// ---------------------------------------------------------------------------------------------
+FFTSO3_real::FFTSO3_real(int l_max_) { init(l_max_); }
+
+void FFTSO3_real::init(int l_max_) { FFTSO3_complex::init(l_max_); }
+
+// T^l: row m < 0 has i/sqrt2 at n = m and -i(-1)^m/sqrt2 at n = -m; row 0 is e_0; row m > 0 has
+// 1/sqrt2 at n = -m and (-1)^m/sqrt2 at n = m (src/fdcl_FFTSO3.cpp:1107-1139)
+FFTSO3_matrix_complex FFTSO3_real::matrix2rsph(int L)
+{
+    FFTSO3_matrix_complex T(L);
+    const double h = 1.0 / std::sqrt(2.0);
+    for (int l = 0; l <= L; l++) {
+        for (int m = -l; m < 0; m++) {
+            const double sm = (m & 1) ? -1.0 : 1.0;
+            T(l, m, m) = I * h;
+            T(l, m, -m) = -I * h * sm;
+        }
+        T(l, 0, 0) = 1.0;
+        for (int m = 1; m <= l; m++) {
+            const double sm = (m & 1) ? -1.0 : 1.0;
+            T(l, m, -m) = h;
+            T(l, m, m) = sm * h;
+        }
+    }
+    return T;
+}
+
+// U^l = conj(T^l) D^l (T^l)^T; T has two nonzeros per row, so each entry is a sum of four terms
+FFTSO3_matrix_real FFTSO3_real::real_harmonics(double alpha, double beta, double gamma, int L)
+{
+    const FFTSO3_matrix_complex D = wigner_D(alpha, beta, gamma, L);
+    const FFTSO3_matrix_complex T = matrix2rsph(L);
+    FFTSO3_matrix_real U(L);
+    for (int l = 0; l <= L; l++)
+        for (int m = -l; m <= l; m++)
+            for (int n = -l; n <= l; n++) {
+                complex<double> u = 0.0;
+                const int as[2] = {m, -m}, bs[2] = {n, -n};
+                for (int ia = 0; ia < (m ? 2 : 1); ia++)
+                    for (int ib = 0; ib < (n ? 2 : 1); ib++)
+                        u += std::conj(T(l, m, as[ia])) * D(l, as[ia], bs[ib]) * T(l, n, bs[ib]);
+                U(l, m, n) = u.real();
+            }
+    return U;
+}
+
+FFTSO3_matrix_real FFTSO3_real::real_harmonics(double alpha, double beta, double gamma)
+{
+    return real_harmonics(alpha, beta, gamma, l_max);
+}
+
+FFTSO3_matrix_real FFTSO3_real::real_harmonics(const Matrix3d &R)
+{
+    const std::vector<double> abg = R2Euler323(R);
+    return real_harmonics(abg[0], abg[1], abg[2], l_max);
+}
+
+FFTSO3_matrix_real FFTSO3_real::forward_transform(std::function<double(double, double, double)> func)
+{
+    const int N = 2 * B;
+    std::vector<double> grid((size_t)N * N * N);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < N; k++)
+        for (int j1 = 0; j1 < N; j1++)
+            for (int j2 = 0; j2 < N; j2++)
+                grid[((size_t)k * N + j1) * N + j2] = func(alpha_j(j1), beta_k(k), gamma_j(j2));
+    return forward_transform(grid);
+}
+
+FFTSO3_matrix_real FFTSO3_real::forward_transform(std::function<double(Matrix3d)> func)
+{
+    return forward_transform([&](double a, double b, double g) { return func(Euler3232R(a, b, g)); });
+}
+
+FFTSO3_matrix_real FFTSO3_real::forward_transform(const std::vector<double> &grid)
+{
+    return forward_transform_batch(grid, 1)[0];
+}
+
+std::vector<FFTSO3_matrix_real> FFTSO3_real::forward_transform_batch(const std::vector<double> &grids, int n_fn)
+{
+    const long N = 2 * B, ng = N * N * N;
+    if (n_fn < 1 || (long)grids.size() != ng * n_fn)
+        throw std::invalid_argument("forward_transform_batch: expected n_fn grids of (2B)^3 values");
+    const long nc = FFTSO3_matrix_real::size_for(l_max);
+    std::vector<double> coef(nc * n_fn);
+    dev_->forward(FFTSO3_DOMAIN_SO3, l_max, n_fn, FFTSO3_REAL, precision_, grids.data(), coef.data());
+    std::vector<FFTSO3_matrix_real> out(n_fn);
+    for (int i = 0; i < n_fn; i++) {
+        out[i].l_max = l_max;
+        out[i].v.assign(coef.begin() + i * nc, coef.begin() + (i + 1) * nc);
+    }
+    return out;
+}
+
+double FFTSO3_real::inverse_transform(FFTSO3_matrix_real F, double alpha, double beta, double gamma)
+{
+    const FFTSO3_matrix_real U = real_harmonics(alpha, beta, gamma, l_max);
+    double f = 0.0;
+    for (int l = 0; l <= l_max; l++)
+        for (int m = -l; m <= l; m++)
+            for (int n = -l; n <= l; n++) f += (double)(2 * l + 1) * F(l, m, n) * U(l, m, n);
+    return f;
+}
+
+double FFTSO3_real::inverse_transform(FFTSO3_matrix_real F, const Matrix3d &R)
+{
+    const std::vector<double> abg = R2Euler323(R);
+    return inverse_transform(std::move(F), abg[0], abg[1], abg[2]);
+}
+
+std::vector<double> FFTSO3_real::inverse_transform(const FFTSO3_matrix_real &F)
+{
+    return inverse_transform_batch(std::vector<FFTSO3_matrix_real>{F});
+}
+
+std::vector<double> FFTSO3_real::inverse_transform_batch(const std::vector<FFTSO3_matrix_real> &F)
+{
+    if (F.empty()) return {};
+    init(F[0].l_max);
+    const long nc = FFTSO3_matrix_real::size_for(l_max), N = 2 * B;
+    std::vector<double> coef(nc * F.size());
+    for (size_t i = 0; i < F.size(); i++) {
+        if (F[i].l_max != l_max) throw std::invalid_argument("inverse_transform_batch: mixed l_max");
+        std::copy(F[i].v.begin(), F[i].v.end(), coef.begin() + i * nc);
+    }
+    std::vector<double> grid(N * N * N * F.size());
+    dev_->inverse(FFTSO3_DOMAIN_SO3, l_max, (int)F.size(), FFTSO3_REAL, precision_, coef.data(), grid.data());
+    return grid;
+}
+
+// the real harmonics are real, orthogonal (U U^T = I) and agree with the explicit d-based form of
+// src/fdcl_FFTSO3.cpp:1062-1105 at degree 1
+double FFTSO3_real::check_real_harmonics()
+{
+    const double a = (double)std::rand() / RAND_MAX * 2 * M_PI, b = (double)std::rand() / RAND_MAX * M_PI,
+                 g = (double)std::rand() / RAND_MAX * 2 * M_PI;
+    const FFTSO3_matrix_real U = real_harmonics(a, b, g, std::max(l_max, 1));
+    double err = 0.0;
+    for (int l = 0; l <= U.l_max; l++) {
+        const auto u = U[l];
+        for (int i = 0; i < 2 * l + 1; i++)
+            for (int j = 0; j < 2 * l + 1; j++) {
+                double s = 0.0;
+                for (int q = 0; q < 2 * l + 1; q++) s += u(i, q) * u(j, q);
+                err = std::max(err, std::fabs(s - (i == j ? 1.0 : 0.0)));
+            }
+    }
+    const FFTSO3_matrix_real d = wigner_d(b, 1);
+    const double r2 = std::sqrt(2.0);
+    // U(1,1,0) = (-1) sqrt2 d(1,1,0) cos(alpha), U(1,0,-1) = -sqrt2 d(1,0,-1) sin(gamma)
+    err = std::max(err, std::fabs(U(1, 1, 0) + r2 * d(1, 1, 0) * std::cos(a)));
+    err = std::max(err, std::fabs(U(1, 0, -1) + r2 * d(1, 0, -1) * std::sin(g)));
+    err = std::max(err, std::fabs(U(0, 0, 0) - 1.0));
+    std::cout << "fdcl::FFTSO3_real::check_real_harmonics: error = " << err << std::endl;
+    return err;
+}
+
+double FFTSO3_real::check_transform()
+{
+    FFTSO3_matrix_real F(l_max);
+    F.setRandom();
+    const FFTSO3_matrix_real Fc = F;
+    FFTSO3_matrix_real F2 = forward_transform(
+        std::function<double(double, double, double)>([&](double a, double b, double g) {
+            return inverse_transform(Fc, a, b, g);
+        }));
+    const double err = (F2 - F).norm();
+    std::cout << "fdcl::FFTSO3_real::check_transform: error = " << err << std::endl;
+    return err;
+}
+
+void FFTSO3_real::check_all()
+{
+    check_weight();
+    check_wigner_d();
+    check_real_harmonics();
+    check_transform();
+}
+
+}  // namespace fdcl

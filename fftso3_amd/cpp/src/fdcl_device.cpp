@@ -1,0 +1,138 @@
+// fdcl_device.cpp -- the class layer's bridge to the C ABI: device plans, host <-> device copies,
+// and the Euler-angle helpers shared by the SO(3) classes.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <string>
+
+#include "fdcl_FFTSO3.hpp"
+
+namespace fdcl
+{
+
+// src/misc_matrix_func.cpp:144-167: beta = acos(R33); at beta = 0 only alpha + gamma is defined
+// and is split evenly
+std::vector<double> R2Euler323(const Matrix3d &R)
+{
+    double a, b, g;
+    b = std::acos(R(2, 2));
+    if (b != 0.) {
+        g = std::atan2(R(2, 1), -R(2, 0));
+        a = std::atan2(R(1, 2), R(0, 2));
+    } else {
+        a = std::acos(R(0, 0)) / 2.;
+        g = a;
+    }
+    return {a, b, g};
+}
+
+// R = exp(a e3) exp(b e2) exp(g e3) (src/misc_matrix_func.cpp:169-178)
+Matrix3d Euler3232R(double a, double b, double g)
+{
+    const double ca = std::cos(a), sa = std::sin(a), cb = std::cos(b), sb = std::sin(b);
+    const double cg = std::cos(g), sg = std::sin(g);
+    Matrix3d R;
+    R(0, 0) = ca * cb * cg - sa * sg;
+    R(0, 1) = -cg * sa - ca * cb * sg;
+    R(0, 2) = ca * sb;
+    R(1, 0) = ca * sg + cb * cg * sa;
+    R(1, 1) = ca * cg - cb * sa * sg;
+    R(1, 2) = sa * sb;
+    R(2, 0) = -cg * sb;
+    R(2, 1) = sb * sg;
+    R(2, 2) = cb;
+    return R;
+}
+
+namespace detail
+{
+
+static void check_abi(int st, const char *what)
+{
+    if (st != FFTSO3_OK)
+        throw std::runtime_error(std::string(what) + ": " + fftso3_status_string(st) + ": " + fftso3_last_error());
+}
+
+static void check_hip(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevicePlan::~DevicePlan()
+{
+    if (plan_) fftso3_plan_destroy(plan_);
+    if (d_grid_) (void)hipFree(d_grid_);
+    if (d_coef_) (void)hipFree(d_coef_);
+}
+
+void DevicePlan::prepare(int domain, int l_max, int n_fn, int kind, int precision)
+{
+    const int key[5] = {domain, l_max, n_fn, kind, precision};
+    if (plan_ && std::memcmp(key, key_, sizeof(key)) == 0) return;
+    if (plan_) {
+        fftso3_plan_destroy(plan_);
+        plan_ = nullptr;
+    }
+    if (d_grid_) (void)hipFree(d_grid_);
+    if (d_coef_) (void)hipFree(d_coef_);
+    d_grid_ = d_coef_ = nullptr;
+    check_abi(fftso3_plan_create(&plan_, domain, l_max, n_fn, kind, precision, 0), "fftso3_plan_create");
+    std::memcpy(key_, key, sizeof(key));
+    grid_elems_ = fftso3_plan_grid_elems(plan_);
+    coef_elems_ = fftso3_plan_coef_elems(plan_);
+    const size_t rsz = precision == FFTSO3_FP64 ? 8 : 4;  // real scalar of device storage
+    const size_t gsz = grid_elems_ * rsz * (kind == FFTSO3_COMPLEX ? 2 : 1);
+    const size_t csz = coef_elems_ * rsz * (kind == FFTSO3_COMPLEX ? 2 : 1);
+    check_hip(hipMalloc(&d_grid_, gsz), "hipMalloc(grid)");
+    check_hip(hipMalloc(&d_coef_, csz), "hipMalloc(coefficients)");
+}
+
+// copy n doubles from host to device storage of precision `prec` (float storage converts)
+static void upload(const double *src, long n, void *dst, int prec, std::vector<char> &stage)
+{
+    if (prec == FFTSO3_FP64) {
+        check_hip(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        return;
+    }
+    stage.resize(n * sizeof(float));
+    float *f = reinterpret_cast<float *>(stage.data());
+    for (long i = 0; i < n; i++) f[i] = (float)src[i];
+    check_hip(hipMemcpy(dst, f, n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+static void download(const void *src, long n, double *dst, int prec, std::vector<char> &stage)
+{
+    if (prec == FFTSO3_FP64) {
+        check_hip(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        return;
+    }
+    stage.resize(n * sizeof(float));
+    float *f = reinterpret_cast<float *>(stage.data());
+    check_hip(hipMemcpy(f, src, n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    for (long i = 0; i < n; i++) dst[i] = f[i];
+}
+
+void DevicePlan::forward(int domain, int l_max, int n_fn, int kind, int precision, const void *grid, void *coef)
+{
+    prepare(domain, l_max, n_fn, kind, precision);
+    const int w = kind == FFTSO3_COMPLEX ? 2 : 1;
+    upload(static_cast<const double *>(grid), grid_elems_ * w, d_grid_, precision, stage_);
+    check_abi(fftso3_forward(plan_, d_grid_, d_coef_, nullptr), "fftso3_forward");
+    check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    download(d_coef_, coef_elems_ * w, static_cast<double *>(coef), precision, stage_);
+}
+
+void DevicePlan::inverse(int domain, int l_max, int n_fn, int kind, int precision, const void *coef, void *grid)
+{
+    prepare(domain, l_max, n_fn, kind, precision);
+    const int w = kind == FFTSO3_COMPLEX ? 2 : 1;
+    upload(static_cast<const double *>(coef), coef_elems_ * w, d_coef_, precision, stage_);
+    check_abi(fftso3_inverse(plan_, d_coef_, d_grid_, nullptr), "fftso3_inverse");
+    check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    download(d_grid_, grid_elems_ * w, static_cast<double *>(grid), precision, stage_);
+}
+
+}  // namespace detail
+}  // namespace fdcl

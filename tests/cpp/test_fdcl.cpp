@@ -1,0 +1,210 @@
+// test_fdcl.cpp -- checks of the fdcl:: class layer, modelled on the reference's own test
+// (test/test.cpp: check_transform round trips at small l_max, 1e-10) and its check_* members.
+//
+//   test_fdcl cpu         host-only checks (weights, Wigner-d, real harmonics, Legendre, layouts)
+//   test_fdcl gpu         transforms through the GPU: round trips, grid vs pointwise inverse
+//   test_fdcl dump BETA L print d^l_mn(BETA) for l <= L, packed order, one value per line
+//   test_fdcl dumpnp X L  print nP^l_m(X) for l <= L, packed order
+//
+// Exit status 0 iff every check passes; one line per check on stdout.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "fdcl_FFTS2.hpp"
+#include "fdcl_FFTSO3.hpp"
+
+static int failures = 0;
+
+static void expect(const char *name, double err, double tol)
+{
+    const bool ok = std::isfinite(err) && err <= tol;
+    std::printf("%s %s err=%.3e tol=%.1e\n", ok ? "PASS" : "FAIL", name, err, tol);
+    if (!ok) failures++;
+}
+
+static int cpu_checks()
+{
+    std::srand(7);
+    // packed layout of the containers: F(l,m,n) at off(l) + (m+l) + (n+l)(2l+1); F[l](m+l, n+l)
+    {
+        fdcl::FFTSO3_matrix_complex F(4);
+        double err = 0;
+        long e = 0;
+        for (int l = 0; l <= 4; l++)
+            for (int n = -l; n <= l; n++)
+                for (int m = -l; m <= l; m++, e++) {
+                    F(l, m, n) = fdcl::complex<double>(l, 10 * m + n);
+                    err += std::abs(F.data()[e] - fdcl::complex<double>(l, 10 * m + n));
+                    err += std::abs(F[l](m + l, n + l) - F(l, m, n));
+                }
+        err += std::fabs((double)e - (double)F.size());
+        expect("so3_matrix_layout", err, 0);
+        fdcl::FFTS2_matrix_real G(5);
+        err = 0;
+        e = 0;
+        for (int l = 0; l <= 5; l++)
+            for (int m = -l; m <= l; m++, e++) {
+                G(l, m) = 100 * l + m;
+                err += std::fabs(G.data()[e] - (100 * l + m)) + std::fabs(G[l](m + l) - G(l, m));
+            }
+        expect("s2_matrix_layout", err, 0);
+    }
+    for (int L : {2, 7, 16}) {
+        fdcl::FFTSO3_complex fc(L);
+        expect(("so3_check_weight_l" + std::to_string(L)).c_str(), fc.check_weight(), 1e-12);
+        expect(("so3_check_wigner_d_l" + std::to_string(L)).c_str(), fc.check_wigner_d(), 1e-12);
+        fdcl::FFTSO3_real fr(L);
+        expect(("so3_check_real_harmonics_l" + std::to_string(L)).c_str(), fr.check_real_harmonics(), 1e-12);
+        fdcl::FFTS2_complex sc(L);
+        expect(("s2_check_weight_l" + std::to_string(L)).c_str(), sc.check_weight(), 1e-12);
+    }
+    // d at beta and -beta: d^l_mn(-beta) = (-1)^(m-n) d^l_mn(beta); wigner_D phases
+    {
+        fdcl::FFTSO3_complex fc(9);
+        const double b = 0.7;
+        const auto d1 = fc.wigner_d(b), d2 = fc.wigner_d(-b);
+        double err = 0;
+        for (int l = 0; l <= 9; l++)
+            for (int m = -l; m <= l; m++)
+                for (int n = -l; n <= l; n++)
+                    err = std::max(err, std::fabs(d2(l, m, n) - (((m - n) & 1) ? -1.0 : 1.0) * d1(l, m, n)));
+        expect("wigner_d_negative_beta", err, 1e-13);
+        const auto D = fc.wigner_D(0.3, b, -1.1);
+        err = 0;
+        for (int l = 0; l <= 9; l++)
+            for (int m = -l; m <= l; m++)
+                for (int n = -l; n <= l; n++)
+                    err = std::max(err, std::abs(D(l, m, n) - std::exp(-fdcl::I * (0.3 * m - 1.1 * n)) * d1(l, m, n)));
+        expect("wigner_D_phases", err, 1e-14);
+        // rotation-matrix overloads agree with the Euler-angle ones
+        const fdcl::Matrix3d R = fdcl::Euler3232R(0.3, b, 1.9);
+        const auto DR = fc.wigner_D(R);
+        const auto DE = fc.wigner_D(0.3, b, 1.9);
+        expect("wigner_D_rotation_overload", (DR - DE).norm(), 1e-12);
+    }
+    // pointwise real inverse equals the real part of the complex inverse of T^H F_r T
+    {
+        const int L = 4;
+        fdcl::FFTSO3_real fr(L);
+        fdcl::FFTSO3_matrix_real Fr(L);
+        Fr.setRandom();
+        const auto T = fr.matrix2rsph(L);
+        fdcl::FFTSO3_matrix_complex Fc(L);
+        for (int l = 0; l <= L; l++)
+            for (int m = -l; m <= l; m++)
+                for (int n = -l; n <= l; n++) {
+                    fdcl::complex<double> s = 0;
+                    for (int a = -l; a <= l; a++)
+                        for (int c = -l; c <= l; c++) s += std::conj(T(l, a, m)) * Fr(l, a, c) * T(l, c, n);
+                    Fc(l, m, n) = s;
+                }
+        fdcl::FFTSO3_complex fc(L);
+        const double fr1 = fr.inverse_transform(Fr, 0.4, 1.2, 2.5);
+        const auto fc1 = fc.inverse_transform(Fc, 0.4, 1.2, 2.5);
+        expect("real_inverse_vs_complex", std::fabs(fr1 - fc1.real()) + std::fabs(fc1.imag()), 1e-12);
+    }
+    return failures;
+}
+
+static double rel(const std::vector<fdcl::complex<double>> &a, const std::vector<fdcl::complex<double>> &b)
+{
+    double n = 0, d = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        n += std::norm(a[i] - b[i]);
+        d += std::norm(b[i]);
+    }
+    return std::sqrt(n / d);
+}
+
+static int gpu_checks()
+{
+    std::srand(11);
+    // the reference's test: round trips at l_max = 5 below 1e-10 (test/test.cpp:39-42,72-88)
+    {
+        fdcl::FFTSO3_complex fc(5);
+        expect("so3_complex_check_transform_l5", fc.check_transform(), 1e-10);
+        fdcl::FFTSO3_real fr(5);
+        expect("so3_real_check_transform_l5", fr.check_transform(), 1e-10);
+        fdcl::FFTS2_complex sc(5);
+        expect("s2_complex_check_transform_l5", sc.check_transform(), 1e-10);
+        fdcl::FFTS2_real sr(5);
+        expect("s2_real_check_transform_l5", sr.check_transform(), 1e-10);
+    }
+    // grid inverse (GPU) against pointwise inverse (host) on a sample of grid points, then the
+    // forward of the grid back to F
+    {
+        const int L = 12, N = 2 * (L + 1);
+        fdcl::FFTSO3_complex fc(L);
+        fdcl::FFTSO3_matrix_complex F(L);
+        F.setRandom();
+        const auto g = fc.inverse_transform(F);
+        double err = 0, nrm = 0;
+        for (int s = 0; s < 40; s++) {
+            const int k = (7 * s) % N, j1 = (13 * s + 1) % N, j2 = (5 * s + 3) % N;
+            const auto p = fc.inverse_transform(F, fc.alpha_j(j1), fc.beta_k(k), fc.gamma_j(j2));
+            err += std::norm(p - g[((size_t)k * N + j1) * N + j2]);
+            nrm += std::norm(p);
+        }
+        expect("so3_grid_inverse_vs_pointwise", std::sqrt(err / nrm), 1e-12);
+        const auto F2 = fc.forward_transform(g);
+        expect("so3_grid_round_trip", rel(F2.v, F.v), 1e-12);
+        // batched entry points and the FP32 device path
+        std::vector<fdcl::FFTSO3_matrix_complex> Fs(3, F);
+        Fs[1] = F * 2.0;
+        Fs[2] = F * fdcl::I;
+        const auto gb = fc.inverse_transform_batch(Fs);
+        const auto Fb = fc.forward_transform_batch(gb, 3);
+        double eb = 0;
+        for (int i = 0; i < 3; i++) eb = std::max(eb, rel(Fb[i].v, Fs[i].v));
+        expect("so3_batch_round_trip", eb, 1e-12);
+        fc.set_precision(FFTSO3_FP32);
+        const auto F3 = fc.forward_transform(fc.inverse_transform(F));
+        expect("so3_fp32_round_trip", rel(F3.v, F.v), 2e-6);
+    }
+    {
+        const int L = 20, N = 2 * (L + 1);
+        fdcl::FFTS2_complex sc(L);
+        fdcl::FFTS2_matrix_complex F(L);
+        F.setRandom();
+        const auto g = sc.inverse_transform(F);
+        double err = 0, nrm = 0;
+        for (int s = 0; s < 40; s++) {
+            const int k = (7 * s) % N, j = (13 * s + 1) % N;
+            const auto p = sc.inverse_transform(F, sc.theta_k(k), sc.phi_j(j));
+            err += std::norm(p - g[(size_t)k * N + j]);
+            nrm += std::norm(p);
+        }
+        expect("s2_grid_inverse_vs_pointwise", std::sqrt(err / nrm), 1e-12);
+        expect("s2_grid_round_trip", rel(sc.forward_transform(g).v, F.v), 1e-12);
+    }
+    return failures;
+}
+
+int main(int argc, char **argv)
+{
+    const std::string mode = argc > 1 ? argv[1] : "cpu";
+    try {
+        if (mode == "cpu") return cpu_checks() ? 1 : 0;
+        if (mode == "gpu") return gpu_checks() ? 1 : 0;
+        if (mode == "dump" && argc > 3) {
+            fdcl::FFTSO3_complex fc(std::atoi(argv[3]));
+            const auto d = fc.wigner_d(std::atof(argv[2]));
+            for (long i = 0; i < d.size(); i++) std::printf("%.17g\n", d.data()[i]);
+            return 0;
+        }
+        if (mode == "dumpnp" && argc > 3) {
+            fdcl::FFTS2_complex sc(1);
+            const auto nP = sc.nor_assoc_Legendre_poly(std::atof(argv[2]), std::atoi(argv[3]));
+            for (long i = 0; i < nP.size(); i++) std::printf("%.17g\n", nP.data()[i]);
+            return 0;
+        }
+    } catch (const std::exception &e) {
+        std::printf("FAIL exception: %s\n", e.what());
+        return 2;
+    }
+    std::fprintf(stderr, "usage: test_fdcl cpu|gpu|dump BETA L|dumpnp X L\n");
+    return 2;
+}
