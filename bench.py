@@ -19,6 +19,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SO(3) fwd+inv transforms/s, B=64 batch 256: split-FP32 vs FP64 @≤1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
+# HBM traffic per launch of each kernel of this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs of this script (written by fftso3_amd.profiling)
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "so3_b64_batch256_fp32_pmc.json")
+# kernels of the batch-wide FP32 path at l_max = 63, by stage
+STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd<false>", "contract_fwd": "contract_fwd_mx32<4, true>",
+                "contract_inv": "contract_inv_mx32<4>", "fft_inv": "so3_fft128_inv<false>"}
 
 
 def log(msg):
@@ -37,7 +43,9 @@ def alg_bytes(B, n_fn, esz, stage):
     return n_fn * per * esz
 
 
-def cpu_baseline(budget_s=12.0):
+def cpu_baseline(budget_s=15.0, max_fn=64):
+    """The FP64 oracle (grid inverse + forward, OpenMP over beta) on the first functions of the same
+    batch, for about budget_s seconds of host time."""
     from oracle import oracle
     from fftso3_amd import inputs
     try:
@@ -46,10 +54,10 @@ def cpu_baseline(budget_s=12.0):
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
     B = 64
-    F = inputs.workload_batch_b64(4)
+    F = inputs.workload_batch_b64(max_fn)
     t0 = time.perf_counter()
     done = 0
-    while done < 4:
+    while done < max_fn:
         g = oracle.so3_inverse(B, F[done], cores)
         oracle.so3_forward(B, g, cores)
         done += 1
@@ -71,6 +79,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed region, copy 1 GiB device to device (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
@@ -113,6 +123,12 @@ def main():
     plan.timing(False)
     value = n_fn * args.steps / dt
     log("fast path: %.1f transforms/s (%.3f ms/step)" % (value, 1e3 * dt / args.steps))
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     # error of the fast path
     rt_err = float(torch.linalg.norm((Fout.to(torch.complex128) - Fd)) / torch.linalg.norm(Fd))
@@ -159,9 +175,16 @@ def main():
     C = plan.chunk()
     bytes_launch = alg_bytes(B, C, 8, dom)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+    traffic = None
+    kname = STAGE_KERNEL[dom] if args.precision == "fp32" and args.chunk is None else dom
+    if kname in STAGE_KERNEL.values() and n_fn == 256 and os.path.exists(PMC_SUMMARY):
+        k = json.load(open(PMC_SUMMARY))["kernels"].get(kname)
+        traffic = k["traffic_bytes"] if k else None
+    roofline = {"bound": "hbm", "kernel": kname, "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms}
+    extra["stage_alg_GBs"] = {s: alg_bytes(B, C, 8, s) / (stages[s][0] / stages[s][1] * 1e-3) / 1e9
+                              for s in ("fft_fwd", "contract_fwd", "contract_inv", "fft_inv")}
     whole = 39146496 * value / 1e9
     extra["pipeline_alg_GBs"] = whole
     extra["pipeline_roofline_frac"] = whole / HBM_PEAK_GBS
@@ -174,7 +197,7 @@ def main():
 
     out = {"metric": METRIC, "value": value, "unit": "transforms/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "batched complex SO(3) transform, l_max=63 (B=64), 256 functions, "
                                   "complex64 grids 256x128^3, coefficients N(0,1/(1+l)^2) seed 2",
                       "l_max": 63, "batch": n_fn, "grid": "256x128^3 complex64", "precision": args.precision},

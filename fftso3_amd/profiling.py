@@ -1,0 +1,103 @@
+"""Summaries of rocprofv3 runs of bench.py: per-kernel average durations and HBM traffic.
+
+Usage (after three profiling runs of the same bench.py command; each summary records them):
+
+    python -m fftso3_amd.profiling STATS_DIR FETCH_DIR WRITE_DIR OUT_PREFIX [--copy-bytes N]
+
+STATS_DIR holds a `rocprofv3 --kernel-trace --stats --output-format csv` run, FETCH_DIR and
+WRITE_DIR `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs.  FETCH_SIZE and WRITE_SIZE
+are read in KiB per dispatch and scaled by the factor that makes the largest device copy of the
+run (bench.py --calibrate-copy: a copy of N bytes) read N bytes and write N bytes.  Writes
+OUT_PREFIX.md (a short table) and OUT_PREFIX.json ({kernel: {avg_ms, fetch_bytes, write_bytes,
+traffic_bytes}}).
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+
+
+def _csvs(d, pattern):
+    return sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+
+
+def kernel_stats(d):
+    """{kernel name: (calls, average ns)} from a --stats run."""
+    out = {}
+    for f in _csvs(d, "*kernel_stats.csv"):
+        for r in csv.DictReader(open(f)):
+            out[r["Name"]] = (int(r["Calls"]), float(r["AverageNs"]))
+    return out
+
+
+def counter_per_dispatch(d, counter):
+    """{kernel name: [value per dispatch]} of one counter from a --pmc run."""
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    for f in _csvs(d, "*counter_collection.csv"):
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] != counter:
+                continue
+            per[r["Kernel_Name"]][int(r["Dispatch_Id"])] += float(r["Counter_Value"])
+    return {k: list(v.values()) for k, v in per.items()}
+
+
+def short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def summarize(stats_dir, fetch_dir, write_dir, copy_bytes):
+    stats = kernel_stats(stats_dir)
+    fetch = counter_per_dispatch(fetch_dir, "FETCH_SIZE")
+    write = counter_per_dispatch(write_dir, "WRITE_SIZE")
+    copies_f = [v for k, vs in fetch.items() if "copyBuffer" in k for v in vs]
+    copies_w = [v for k, vs in write.items() if "copyBuffer" in k for v in vs]
+    if not copies_f or not copies_w:
+        raise SystemExit("no device copy in the PMC runs: run bench.py with --calibrate-copy")
+    f_fetch = copy_bytes / (max(copies_f) * 1024.0)
+    f_write = copy_bytes / (max(copies_w) * 1024.0)
+    table = {}
+    for k in sorted(set(fetch) | set(write)):
+        if "copyBuffer" in k:
+            continue
+        fv, wv = fetch.get(k, [0.0]), write.get(k, [0.0])
+        fb = sum(fv) / len(fv) * 1024.0 * f_fetch
+        wb = sum(wv) / len(wv) * 1024.0 * f_write
+        calls, avg_ns = stats.get(k, (0, float("nan")))
+        table[short(k)] = {"kernel": k, "calls": calls, "avg_ms": avg_ns * 1e-6, "fetch_bytes": fb,
+                           "write_bytes": wb, "traffic_bytes": fb + wb}
+    return {"fetch_scale": f_fetch, "write_scale": f_write, "copy_bytes": copy_bytes, "kernels": table}
+
+
+def to_markdown(s, command):
+    lines = ["| kernel | calls | avg ms | HBM read (GB) | HBM write (GB) | traffic (GB) |",
+             "|---|---|---|---|---|---|"]
+    for k, v in sorted(s["kernels"].items(), key=lambda kv: -(kv[1]["avg_ms"] * max(kv[1]["calls"], 1))):
+        lines.append("| %s | %d | %.3f | %.3f | %.3f | %.3f |" % (
+            k, v["calls"], v["avg_ms"], v["fetch_bytes"] / 1e9, v["write_bytes"] / 1e9, v["traffic_bytes"] / 1e9))
+    head = ["Command:", "", "```", command, "```", "",
+            "FETCH_SIZE scaled by %.3f and WRITE_SIZE by %.3f (a %d-byte device copy read and written)." % (
+                s["fetch_scale"], s["write_scale"], s["copy_bytes"]), ""]
+    return "\n".join(head + lines) + "\n"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("stats_dir")
+    ap.add_argument("fetch_dir")
+    ap.add_argument("write_dir")
+    ap.add_argument("out_prefix")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--command", default="")
+    a = ap.parse_args()
+    s = summarize(a.stats_dir, a.fetch_dir, a.write_dir, a.copy_bytes)
+    with open(a.out_prefix + ".json", "w") as f:
+        json.dump(s, f, indent=1)
+    with open(a.out_prefix + ".md", "w") as f:
+        f.write(to_markdown(s, a.command))
+    print(to_markdown(s, a.command))
+
+
+if __name__ == "__main__":
+    main()
