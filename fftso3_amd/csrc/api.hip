@@ -88,10 +88,8 @@ struct Class {
     int row_tile = 0, col_tile = 0; // two-pass fallback tiles
     bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
     int chunk = 8;                  // functions per pipeline pass for this class
-    DevBuf d_items_mx, d_goff;      // groups (largest first) and their first row in CG
-    DevBuf d_slot;                  // packed coefficient index -> CG row (pair-major)
+    DevBuf d_items_mx;              // groups of the batch-wide path (largest first)
     int nitems_mx = 0;
-    long cg_rows = 0;               // rows of the group-major coefficient staging array
 };
 
 }  // namespace
@@ -291,20 +289,12 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     c->nitems_fwd_mx = (int)mf.size();
     c->nitems_inv_mx = (int)mi.size();
     if ((st = c->d_items_fwd_mx.upload(mf)) || (st = c->d_items_inv_mx.upload(mi))) return st;
-    // batch-wide MFMA path: groups ordered by decreasing row count; CG row offsets per group
-    std::vector<int> mx(ngrp), goff(ngrp);
-    long rows = 0;
-    for (int g = 0; g < ngrp; g++) {
-        int m0 = s2 ? g : (int)std::sqrt((double)g + 0.5);
-        if (!s2) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
-        goff[g] = (int)rows;
-        rows += B - m0;
-        mx[g] = g;
-    }
-    // group g has m0 = floor(sqrt(g)) (SO(3)) or g (S^2): ascending g is descending row count
-    c->cg_rows = rows;
+    // batch-wide MFMA path: one workgroup per group; group g has m0 = floor(sqrt(g)) (SO(3)) or g
+    // (S^2), so ascending g is descending row count (largest first)
+    std::vector<int> mx(ngrp);
+    for (int g = 0; g < ngrp; g++) mx[g] = g;
     c->nitems_mx = ngrp;
-    if ((st = c->d_items_mx.upload(mx)) || (st = c->d_goff.upload(goff))) return st;
+    if ((st = c->d_items_mx.upload(mx))) return st;
     const int CC = c->chunk;
     std::vector<int> ident(CC);
     for (int i = 0; i < CC; i++) ident[i] = i;
