@@ -69,6 +69,122 @@ def cpu_baseline(budget_s=15.0, max_fn=64):
                       "(OpenMP over beta, %d threads), %.1f s" % (done, cores, dt)}
 
 
+OTHER_WORKLOADS = {
+    "so3_uneven512_real": "512 real SO(3) functions, bandlimits uniform in {8,16,24,32,48,64} (seed 3), "
+                          "coefficients N(0,1/(1+l)^2), float32",
+    "so3_b256x8": "batched complex SO(3) transform, l_max=255 (B=256), 8 functions, complex64 grids 8x512^3, "
+                  "coefficients N(0,1/(1+l)^2) seed 4",
+    "s2_b128x4096": "batched complex S^2 transform, l_max=127 (B=128), 4096 fields, complex64 grids 4096x256^2, "
+                    "coefficients N(0,1/(1+l)^2) seed 5",
+}
+
+
+def workload_bytes(domain, Bs, real):
+    """Algorithmic HBM bytes of one fwd+inv step per stage group: grids, unique spectra and
+    coefficients of every function, each read once and written once."""
+    esz = 4 if real else 8
+    grid = spec = coef = 0
+    for B in Bs:
+        N = 2 * B
+        if domain == "s2":
+            grid += N * N * esz
+            spec += N * (2 * B - 1) * 8
+            coef += B * B * esz
+        else:
+            grid += N ** 3 * esz
+            spec += N * (2 * B - 1) ** 2 * 8
+            coef += B * (4 * B * B - 1) // 3 * esz
+    return {"fft_fwd": grid + spec, "contract_fwd": spec + coef + (coef if real else 0),
+            "contract_inv": coef + spec, "fft_inv": spec + grid}
+
+
+def bench_other(args, name):
+    """The other workloads of the survey's configuration list, one JSON line each (FP32)."""
+    import torch
+    import fftso3_amd
+    from fftso3_amd import inputs
+    from oracle import oracle
+    dev = torch.device("cuda:0")
+    if name == "so3_uneven512_real":
+        Bs, coefs = inputs.workload_uneven(512)
+        plan = fftso3_amd.Plan([B - 1 for B in Bs], domain="so3", kind="real", precision="fp32")
+        F = torch.from_numpy(np.concatenate(coefs)).to(torch.float32).to(dev)
+        domain, real = "so3", True
+        ref_fn = lambda i: (oracle.so3_real_forward(Bs[i], oracle.so3_real_inverse(Bs[i], coefs[i], 0), 0))
+    elif name == "so3_b256x8":
+        Bs = [256] * 8
+        Fh = inputs.workload_b256(8)
+        plan = fftso3_amd.Plan(255, 8, "so3", "complex", "fp32")
+        F = torch.from_numpy(Fh.reshape(-1)).to(torch.complex64).to(dev)
+        domain, real = "so3", False
+        ref_fn = lambda i: oracle.so3_forward(256, oracle.so3_inverse(256, Fh[i], 0), 0)
+    else:
+        Bs = [128] * 4096
+        Fh = inputs.workload_s2(4096)
+        plan = fftso3_amd.Plan(127, 4096, "s2", "complex", "fp32")
+        F = torch.from_numpy(Fh.reshape(-1)).to(torch.complex64).to(dev)
+        domain, real = "s2", False
+        ref_fn = lambda i: oracle.s2_forward(128, oracle.s2_inverse(128, Fh[i]))
+    n_fn = len(Bs)
+    gdt, cdt = plan.dtypes()
+    grid = torch.empty(plan.grid_elems(), dtype=gdt, device=dev)
+    Fout = torch.empty_like(F)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        plan.inverse(F, grid, stream)
+        plan.forward(grid, Fout, stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    plan.timing(True)
+    plan.timing_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = plan.timing_read()
+    plan.timing(False)
+    value = n_fn * args.steps / dt
+    log("%s: %.1f transforms/s (%.3f ms/step)" % (name, value, 1e3 * dt / args.steps))
+    a = Fout.to(torch.complex128 if not real else torch.float64)
+    b = F.to(a.dtype)
+    rt = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    per_stage = workload_bytes(domain, Bs, real)
+    dom = max(per_stage, key=lambda st: stages[st][0])
+    avg_ms = stages[dom][0] / stages[dom][1]
+    # a stage may be several launches (one per bandlimit class); bytes are per launch on average
+    bl = per_stage[dom] / (stages[dom][1] / args.steps)
+    achieved = bl / (avg_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        done = 0
+        while done < n_fn and time.perf_counter() - t0 < 15.0:
+            ref_fn(done)
+            done += 1
+        ct = time.perf_counter() - t0
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cpu = {"value": done / ct, "unit": "transforms/s", "cores": cores, "kind": "port",
+               "sample": "first %d functions of the workload, FP64 oracle inverse + forward, %.1f s" % (done, ct)}
+    out = {"metric": "%s fwd+inv transforms/s (FP32)" % name, "value": value, "unit": "transforms/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": OTHER_WORKLOADS[name], "functions": n_fn},
+           "roofline": {"bound": "hbm", "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": bl,
+                        "avg_launch_ms": avg_ms},
+           "cpu_baseline": cpu,
+           "extra": {"roundtrip_rel_l2": rt,
+                     "stage_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
+                     "stage_launches_per_step": {k: v[1] / args.steps for k, v in stages.items()},
+                     "pipeline_alg_GBs": sum(per_stage.values()) * value / n_fn / 1e9}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,11 +197,16 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB device to device (PMC calibration)")
+    ap.add_argument("--workload", default="so3_b64x256", choices=["so3_b64x256"] + sorted(OTHER_WORKLOADS),
+                    help="so3_b64x256 is the headline; the others print their own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload != "so3_b64x256":
+        bench_other(args, args.workload)
+        return
 
     import torch
     import fftso3_amd

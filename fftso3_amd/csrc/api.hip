@@ -270,9 +270,11 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
         if (!s2) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
         for (int l = m0; l < B; l += c->lt_fwd) itf.push_back(make_int2(g, l));
     }
-    // inverse: k tile sized so that D[B][KT] + Fv[B][ncol] stays within 64 KB of LDS
+    // inverse: k tile sized so that D[B][KT] + Fv[B][ncol] fits the LDS (every k tile re-reads
+    // the group's coefficients, so larger tiles at large B matter more than occupancy)
     int kt = N;
-    while (kt > 4 && (size_t)B * (kt + ncol) * tsz > 64 * 1024) kt /= 2;
+    const size_t inv_lds_cap = B > 64 ? (size_t)kMaxLds - 8 * 1024 : (size_t)64 * 1024;
+    while (kt > 4 && (size_t)B * (kt + ncol) * tsz > inv_lds_cap) kt /= 2;
     c->kt_inv = kt;
     for (int g = 0; g < ngrp; g++)
         for (int k = 0; k < N; k += kt) iti.push_back(make_int2(g, k));
