@@ -87,6 +87,7 @@ struct Class {
     bool fused = false;             // whole slice fits in LDS
     int row_tile = 0, col_tile = 0; // two-pass fallback tiles
     bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
+    bool mxk = false;               // ... with the K-chunked kernels (128 < N, B <= 128)
     int chunk = 8;                  // functions per pipeline pass for this class
     DevBuf d_items_mx;              // groups of the batch-wide path (largest first)
     int nitems_mx = 0;
@@ -252,7 +253,15 @@ int build_class_tables(fftso3_plan_s *pl, Class *c)
 int build_class_work(fftso3_plan_s *pl, Class *c)
 {
     const int B = c->B, N = c->N;
-    c->mx = pl->prec == FFTSO3_FP32 && N <= 128 && !pl->force_chunked;
+    // batch-wide FP32 MFMA contraction: N <= 128 (D in registers / LDS), or up to B = 128 with the
+    // group's d block resident in LDS (K-chunked kernels)
+    {
+        const int np = pl->domain == FFTSO3_DOMAIN_S2 ? 2 : 4;
+        const bool fits = B <= MXK_MAXROWS && mxk_lds_bytes(N, B, np, true) <= (size_t)kMaxLds &&
+                          mxk_lds_bytes(N, B, np, false) <= (size_t)kMaxLds;
+        c->mxk = pl->prec == FFTSO3_FP32 && N > 128 && fits && !pl->force_chunked;
+        c->mx = (pl->prec == FFTSO3_FP32 && N <= 128 && !pl->force_chunked) || c->mxk;
+    }
     c->chunk = c->mx ? (int)c->fns.size() : pl->C;
     const int C = c->mx ? 8 : c->chunk;  // tiling of the chunked kernels (unused by the MX path)
     const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
@@ -545,7 +554,10 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             StageTimer t(pl, s, 1);
             a.src = pl->d_ws.p;
             a.dst = pl->d_fc.p;
-            if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+            if (c->mxk)
+                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(xcd_grid(a.nitems)), dim3(256),
+                                   mxk_lds_bytes(c->N, c->B, NP, true), s, a);
+            else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
             else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
             HIP_TRY(hipGetLastError());
         }
@@ -573,7 +585,10 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         StageTimer t(pl, s, 4);
         a.src = pl->d_fc.p;
         a.dst = pl->d_ws.p;
-        hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+        if (c->mxk)
+            hipLaunchKernelGGL((contract_inv_mxk<NP>), dim3(xcd_grid(a.nitems)), dim3(256),
+                               mxk_lds_bytes(c->N, c->B, NP, false), s, a);
+        else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
         HIP_TRY(hipGetLastError());
     }
     return FFTSO3_OK;
@@ -795,6 +810,10 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32<4, false>));
     ALLOW_LDS((contract_fwd_mx32<2, false>));
     ALLOW_LDS((contract_inv_mx32<4>));
+    ALLOW_LDS((contract_fwd_mxk<4>));
+    ALLOW_LDS((contract_fwd_mxk<2>));
+    ALLOW_LDS((contract_inv_mxk<4>));
+    ALLOW_LDS((contract_inv_mxk<2>));
     ALLOW_LDS((contract_inv_mx32<2>));
     ALLOW_LDS((so3_fft128_fwd<true>));
     ALLOW_LDS((so3_fft128_fwd<false>));

@@ -205,3 +205,27 @@ def test_s2_split_bf16_b128(fftso3, torch_cuda, oracle, prec):
     for i in range(4):
         assert rel_l2(g[i], oracle.s2_inverse(B, F[i])) < SPLIT_TOL[prec]
         assert rel_l2(c[i], F[i]) < SPLIT_TOL[prec]
+
+
+@pytest.mark.parametrize("domain,B", [("s2", 75), ("s2", 100), ("so3", 70), ("so3", 100), ("so3", 128)])
+def test_batchwide_large_n_fp32(fftso3, torch_cuda, oracle, domain, B):
+    """FP32 batch-wide contraction with the d block resident in LDS (128 < N <= 256), including N
+    that are not multiples of 32; 3 functions (S^2: 5) against the oracle, complex and real."""
+    n = 5 if domain == "s2" else 3
+    if domain == "s2":
+        F = inputs.s2_coefficients(B, n, 800 + B)
+        g, c = s2_roundtrip(fftso3, torch_cuda, B, F, prec="fp32")
+        for i in range(n):
+            assert rel_l2(g[i], oracle.s2_inverse(B, F[i])) < FP32_TOL
+            assert rel_l2(c[i], F[i]) < FP32_TOL
+        Fr = inputs.s2_coefficients(B, 2, 900 + B, complex_=False)
+        g, c = s2_roundtrip(fftso3, torch_cuda, B, Fr, kind="real", prec="fp32")
+        for i in range(2):
+            assert rel_l2(g[i], oracle.s2_real_inverse(B, Fr[i])) < FP32_TOL
+            assert rel_l2(c[i], Fr[i]) < FP32_TOL
+    else:
+        F = inputs.so3_coefficients(B, n, 800 + B, scale=lambda l: 1.0 / (1.0 + l))
+        g, c = so3_roundtrip(fftso3, torch_cuda, B, F, prec="fp32")
+        for i in range(n):
+            assert rel_l2(g[i], oracle.so3_inverse(B, F[i])) < FP32_TOL
+            assert rel_l2(c[i], F[i]) < FP32_TOL
