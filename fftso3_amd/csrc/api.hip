@@ -537,6 +537,12 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     a.rt = rec_tables(c);
     a.items = (const int *)c->d_items_mx.p;
     a.nitems = c->nitems_mx;
+    {
+        // split each group's column blocks over several workgroups when there are few groups
+        const int nfb = (ca.nfc * NP * 2 + 127) / 128;
+        a.nsplit = std::max(1, std::min(nfb, (2048 + a.nitems - 1) / a.nitems));
+    }
+    const int mgrid = xcd_grid(a.nitems * a.nsplit);
     const size_t lds = MX_LDS_BYTES;
     a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);
     const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
@@ -555,10 +561,10 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             a.src = pl->d_ws.p;
             a.dst = pl->d_fc.p;
             if (c->mxk)
-                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(xcd_grid(a.nitems)), dim3(256),
+                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(mgrid), dim3(256),
                                    mxk_lds_bytes(c->N, c->B, NP, true), s, a);
-            else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
-            else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+            else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
+            else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
             HIP_TRY(hipGetLastError());
         }
         StageTimer t(pl, s, 2);
@@ -586,9 +592,9 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         a.src = pl->d_fc.p;
         a.dst = pl->d_ws.p;
         if (c->mxk)
-            hipLaunchKernelGGL((contract_inv_mxk<NP>), dim3(xcd_grid(a.nitems)), dim3(256),
+            hipLaunchKernelGGL((contract_inv_mxk<NP>), dim3(mgrid), dim3(256),
                                mxk_lds_bytes(c->N, c->B, NP, false), s, a);
-        else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(xcd_grid(a.nitems)), dim3(256), lds, s, a);
+        else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
         HIP_TRY(hipGetLastError());
     }
     return FFTSO3_OK;

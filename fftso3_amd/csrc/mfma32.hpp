@@ -30,12 +30,28 @@ typedef float f32x16m_t __attribute__((ext_vector_type(16)));
 struct MxArgs {
     ChunkArgs ca;            // ca.C = functions of the class (workspace stride)
     RecTables rt;
-    const int *items;        // group per workgroup (largest first)
+    const int *items;        // groups (largest first)
     int nitems;
+    int nsplit;              // workgroups per group, each taking a contiguous range of column blocks
     const void *src;         // forward: spectrum workspace; inverse: PT
     void *dst;               // forward: PT; inverse: spectrum workspace
     float *dummy;            // 16-byte slot that absorbs the stores of lanes with nothing to store
 };
+
+// Work unit of a batch-wide kernel: group a.items[u / nsplit] and its column blocks [fb0, fb1)
+// (nfb blocks of 128 spectrum columns in all).  False for units past the grid or with no blocks;
+// either is uniform over the workgroup.
+__device__ __forceinline__ bool mx_unit(const MxArgs &a, int nfb, int &g, int &fb0, int &fb1)
+{
+    const int n = a.nitems * a.nsplit;
+    const int unit = xcd_unit(n);
+    if (unit >= n) return false;
+    const int it = unit / a.nsplit, part = unit - it * a.nsplit;
+    g = a.items[it];
+    fb0 = (int)((long)part * nfb / a.nsplit);
+    fb1 = (int)((long)(part + 1) * nfb / a.nsplit);
+    return fb0 < fb1;
+}
 
 template <int NP>
 __device__ __forceinline__ void mx_group(int g, int &m0, int &n0)
@@ -89,16 +105,15 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     float *L = D + 64 * MX_DPITCH;                       // [128][MX_LPITCH], spectrum block
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
-    const int unit = xcd_unit(a.nitems);
-    if (unit >= a.nitems) return;
-    const int g = a.items[unit];
+    const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
+    int g, fb0, fb1;
+    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nrt = (M + 31) / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     constexpr int FB = 64 / NP;                          // functions per block
-    const int nfb = (ca.nfc + FB - 1) / FB;
     // load role: rows r0 + 8i (i < 16) of the block, columns 4q..4q+3
     const int q = tid & 31, r0 = tid >> 5;
     const int fq = (2 * q) / NP;                         // function (within the block) of the 4 columns
@@ -113,7 +128,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
 #pragma unroll
         for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + roff[i] + co);
     };
-    load_block(0);
+    load_block(fb0);
     const int N8 = (N + 7) & ~7;                         // beta extent multiplied (zero columns past N)
     for (int k = tid; k < N8; k += 256) {
         if (k < N) {
@@ -143,7 +158,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
             dB[s] = nrt > 1 ? D0[32 * MX_DPITCH + 2 * s] : 0.f;
         }
     }
-    for (int fb = 0; fb < nfb; fb++) {
+    for (int fb = fb0; fb < fb1; fb++) {
         __syncthreads();                                 // D written / previous block consumed
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -152,7 +167,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
             *reinterpret_cast<float2 *>(row + lc1) = make_float2(pf[i].z, pf[i].w);
         }
         __syncthreads();
-        if (fb + 1 < nfb) load_block(fb + 1);
+        if (fb + 1 < fb1) load_block(fb + 1);
         f32x16m_t acc0, acc1;
 #pragma unroll
         for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
@@ -235,16 +250,15 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
     float *O = D + 64 * MX_DPITCH;                       // [128][MX_LPITCH], output block
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
-    const int unit = xcd_unit(a.nitems);
-    if (unit >= a.nitems) return;
-    const int g = a.items[unit];
+    const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
+    int g, fb0, fb1;
+    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, M4 = (M + 3) & ~3, nkt = (N + 31) / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     constexpr int FB = 64 / NP;
-    const int nfb = (ca.nfc + FB - 1) / FB;
     // operand role of this lane: pair p, function fo of the block, re/im part cl & 1
     const int p = __builtin_amdgcn_readfirstlane(wave) % NP, fo = (wave / NP) * 16 + (cl >> 1);
     const float *PT = reinterpret_cast<const float *>(a.src);
@@ -259,7 +273,7 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
 #pragma unroll
         for (int s = 0; s < 32; s++) buf[s] = src[2L * eoff[s] * C];
     };
-    load_block(bA, 0);
+    load_block(bA, fb0);
     for (int k = tid; k < 32 * nkt; k += 256) {
         if (k < N) {
             group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
@@ -342,11 +356,11 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
         }
     };
     __syncthreads();                                     // D written
-    for (int fb = 0; fb < nfb; fb += 2) {
-        if (fb + 1 < nfb) load_block(bB, fb + 1);
+    for (int fb = fb0; fb < fb1; fb += 2) {
+        if (fb + 1 < fb1) load_block(bB, fb + 1);
         multiply_store(bA, fb);
-        if (fb + 1 >= nfb) break;
-        if (fb + 2 < nfb) load_block(bA, fb + 2);
+        if (fb + 1 >= fb1) break;
+        if (fb + 2 < fb1) load_block(bA, fb + 2);
         multiply_store(bB, fb + 1);
     }
 }
@@ -428,9 +442,9 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
-    const int unit = xcd_unit(a.nitems);
-    if (unit >= a.nitems) return;
-    const int g = a.items[unit];
+    const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
+    int g, fb0, fb1;
+    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int N32 = (N + 31) & ~31, NK = N32 / 32;
@@ -440,7 +454,6 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
     int *etab = reinterpret_cast<int *>(L + MXK_LCHUNK * MX_LPITCH); // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
-    const int nfb = (NCOL + 127) / 128;
     const float *S = reinterpret_cast<const float *>(a.src);
     // load role: rows r0 + 8i (i < 4) of a chunk, columns 4q..4q+3 of the block
     const int q = tid & 31, r0 = tid >> 5;
@@ -455,7 +468,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
         pf2 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_ + 16, N - 1)) * CW + co_); \
         pf3 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_ + 24, N - 1)) * CW + co_); \
     }
-    MXK_LOAD_CHUNK(0, 0)
+    MXK_LOAD_CHUNK(fb0, 0)
     for (int k = tid; k < N32; k += 256) {
         if (k < N) {
             const double wk = a.rt.w[k];
@@ -477,7 +490,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
     const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
     const bool pvalid = mx_pair_valid<NP>(m0, n0, p);
     float *PT = reinterpret_cast<float *>(a.dst);
-    for (int fb = 0; fb < nfb; fb++) {
+    for (int fb = fb0; fb < fb1; fb++) {
         f32x16m_t acc[4];
 #pragma unroll
         for (int t = 0; t < 4; t++)
@@ -492,7 +505,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
             __syncthreads();
             {
                 const int nf = kc + 1 < NK ? fb : fb + 1, nk = kc + 1 < NK ? kc + 1 : 0;
-                if (nf < nfb) MXK_LOAD_CHUNK(nf, nk)  // the next chunk streams in during this one
+                if (nf < fb1) MXK_LOAD_CHUNK(nf, nk)  // the next chunk streams in during this one
             }
             // rows k >= N of a chunk hold finite copies of row N-1 and meet zero columns of D
             const float *Lw = L + kh * MX_LPITCH + cb;
@@ -532,9 +545,9 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
-    const int unit = xcd_unit(a.nitems);
-    if (unit >= a.nitems) return;
-    const int g = a.items[unit];
+    const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
+    int g, fb0, fb1;
+    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, M4 = (M + 3) & ~3, DP = N + 1, nkt = (N + 31) / 32;
@@ -542,7 +555,6 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
     int *etab = reinterpret_cast<int *>(D + (size_t)M4 * DP);       // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
-    const int nfb = (NCOL + 127) / 128;
     for (int i = tid; i < NP * MXK_MAXROWS; i += 256) {
         const int p = i / MXK_MAXROWS, r = i - p * MXK_MAXROWS;
         etab[i] = MxPair<NP>(m0, n0, p).packed(m0 + min(r, M - 1));
@@ -563,7 +575,7 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
     const float *PT = reinterpret_cast<const float *>(a.src);
     float *G = reinterpret_cast<float *>(a.dst);
     const int *et = etab + p * MXK_MAXROWS;
-    for (int fb = 0; fb < nfb; fb++) {
+    for (int fb = fb0; fb < fb1; fb++) {
         const int c = fb * 128 + cb, cv = c < NCOL ? c : 0;
         const float *src = PT + 2 * (cv / (2 * NP)) + (cv & 1);
         // coefficient rows l = m0 + 2s + kh (clamped to finite rows; zero rows of D beyond M)
