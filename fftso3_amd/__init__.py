@@ -29,6 +29,9 @@ _SIGS = {
     "fftso3_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fftso3_plan_set_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_plan_get_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "fftso3_plan_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fftso3_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int]),
     "fftso3_plan_grid_offset": (ctypes.c_long, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_plan_coef_offset": (ctypes.c_long, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_plan_grid_elems": (ctypes.c_long, [ctypes.c_void_p]),
@@ -184,6 +187,23 @@ class Plan:
         _check(lib().fftso3_inverse(self._h, ctypes.c_void_p(coef.data_ptr()), ctypes.c_void_p(grid.data_ptr()),
                                     self._stream(stream)))
         return grid
+
+    def set_graph(self, on=True):
+        """Capture each (direction, input, output, stream) into a hipGraph and replay it."""
+        _check(lib().fftso3_plan_set_graph(self._h, 1 if on else 0))
+
+    def autotune(self, grid=None, coef=None, stream=None, reps=2):
+        """Pick the fastest schedule for this plan; grid / coef (device tensors) are clobbered."""
+        import torch
+        gdt, cdt = self.dtypes()
+        if grid is None:
+            grid = torch.empty(self.grid_elems(), dtype=gdt, device="cuda")
+        if coef is None:
+            coef = torch.zeros(self.coef_elems(), dtype=cdt, device="cuda")
+        self._validate(grid, self.grid_elems(), gdt, "grid")
+        self._validate(coef, self.coef_elems(), cdt, "coefficients")
+        _check(lib().fftso3_plan_autotune(self._h, ctypes.c_void_p(grid.data_ptr()), ctypes.c_void_p(coef.data_ptr()),
+                                          self._stream(stream), int(reps)))
 
     # --- timing ---
     def timing(self, on=True):

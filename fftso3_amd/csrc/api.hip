@@ -111,6 +111,23 @@ struct fftso3_plan_s {
     DevBuf d_dummy;     // store sink of the batch-wide contraction (lanes with nothing to store)
     DevBuf d_tmp;       // packed complex coefficients of the batch-wide real path
     size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0, tmp_bytes = 0;
+    int klo = 0;                       // row blocking of the batch-wide workspace (ws_row)
+    int split_mul = 1;                 // multiplier of the automatic group split (batch-wide kernels)
+    // graph mode: one instantiated graph per (direction, input, output, stream)
+    bool graph = false;
+    struct GraphEntry {
+        bool fwd;
+        const void *in;
+        void *out;
+        hipStream_t s;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    void drop_graphs()
+    {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -119,6 +136,7 @@ struct fftso3_plan_s {
     long acc_n[kNumStages] = {};
     ~fftso3_plan_s()
     {
+        drop_graphs();
         for (auto *c : classes) delete c;
         for (auto &e : ev_pool) {
             (void)hipEventDestroy(e.first);
@@ -338,6 +356,7 @@ static int ws_klo()
 
 int finalize_plan(fftso3_plan_s *pl)
 {
+    pl->drop_graphs();  // captured graphs hold the old buffers
     int st;
     for (auto *c : pl->classes) delete c;
     pl->classes.clear();
@@ -360,7 +379,7 @@ int finalize_plan(fftso3_plan_s *pl)
         if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
         if ((st = c->d_fn.upload(c->fns))) return st;
         const size_t B = c->B, N = c->N, C = c->chunk;
-        const int klo = ws_klo();
+        const int klo = pl->klo;
         size_t w = (size_t)ws_rows(c->mx, klo, s2 ? (int)B : (int)(B * B), (int)N) * C * (s2 ? 2 : 4) * pl->esz;
         ws = std::max(ws, w);
         if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
@@ -442,7 +461,7 @@ ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
     ca.grid_off = reinterpret_cast<const long *>(pl->d_grid_off.p);
     ca.coef_off = reinterpret_cast<const long *>(pl->d_coef_off.p);
     ca.kmajor = c->mx ? 1 : 0;
-    ca.klo = ws_klo();
+    ca.klo = pl->klo;
     ca.ngrp = pl->domain == FFTSO3_DOMAIN_S2 ? c->B : c->B * c->B;
     return ca;
 }
@@ -540,7 +559,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     {
         // split each group's column blocks over several workgroups when there are few groups
         const int nfb = (ca.nfc * NP * 2 + 127) / 128;
-        a.nsplit = std::max(1, std::min(nfb, (2048 + a.nitems - 1) / a.nitems));
+        a.nsplit = std::max(1, std::min(nfb, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
     }
     const int mgrid = xcd_grid(a.nitems * a.nsplit);
     const size_t lds = MX_LDS_BYTES;
@@ -859,6 +878,7 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
     pl->prec = precision;
     pl->n_fn = n_fn;
     pl->esz = precision == FFTSO3_FP64 ? 16 : 8;
+    pl->klo = ws_klo();
     pl->Bs.resize(n_fn);
     pl->grid_off.resize(n_fn);
     pl->coef_off.resize(n_fn);
@@ -963,10 +983,103 @@ static int run_dispatch(fftso3_plan_t plan, bool fwd, const void *in, void *out,
     if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
     if (!in || !out) return fail(FFTSO3_EINVAL, "NULL device pointer");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    switch (plan->prec) {
-    case FFTSO3_FP64: return run<double>(plan, fwd, in, out, s);
-    default: return run<float>(plan, fwd, in, out, s);
+    auto eager = [&]() {
+        return plan->prec == FFTSO3_FP64 ? run<double>(plan, fwd, in, out, s) : run<float>(plan, fwd, in, out, s);
+    };
+    // graph mode needs a capturable stream and no per-stage timing events
+    if (!plan->graph || s == nullptr || plan->timing) return eager();
+    for (auto &g : plan->graphs)
+        if (g.fwd == fwd && g.in == in && g.out == out && g.s == s) {
+            HIP_TRY(hipGraphLaunch(g.exec, s));
+            return FFTSO3_OK;
+        }
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = eager();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (st != FFTSO3_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
     }
+    if (ec != hipSuccess) return fail(FFTSO3_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(FFTSO3_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    if (plan->graphs.size() >= 16) {
+        (void)hipGraphExecDestroy(plan->graphs.front().exec);
+        plan->graphs.erase(plan->graphs.begin());
+    }
+    plan->graphs.push_back({fwd, in, out, s, exec});
+    HIP_TRY(hipGraphLaunch(exec, s));
+    return FFTSO3_OK;
+}
+
+int fftso3_plan_set_graph(fftso3_plan_t plan, int on)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    plan->graph = on != 0;
+    if (!plan->graph) plan->drop_graphs();
+    return FFTSO3_OK;
+}
+
+// Times inverse + forward (reps each after one warm-up) under each candidate schedule and keeps
+// the fastest: for batch-wide classes the workspace row blocking and the group split; for the
+// chunked kernels the chunk size.  The two arrays serve as scratch.
+int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (!d_grid || !d_coef) return fail(FFTSO3_EINVAL, "NULL device pointer");
+    if (reps < 1 || reps > 100) return fail(FFTSO3_EINVAL, "reps must be in [1, 100] (got %d)", reps);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    struct Cand { bool chunked; int C, klo, split; };
+    std::vector<Cand> cands;
+    const bool bf16 = plan->prec == FFTSO3_BF16X3 || plan->prec == FFTSO3_BF16X6;
+    if (!bf16) {
+        if (plan->prec == FFTSO3_FP32)
+            for (int klo : {0, 3})
+                for (int sp : {1, 2, 4}) cands.push_back({false, plan->C, klo, sp});
+        for (int C : {1, 2, 4, 8}) cands.push_back({true, C, plan->klo, 1});
+    }
+    if (cands.empty()) return FFTSO3_OK;
+    const bool timing = plan->timing, graph = plan->graph;
+    plan->timing = false;
+    plan->graph = false;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand win = cands[0];
+    int st = FFTSO3_OK;
+    for (const Cand &c : cands) {
+        plan->force_chunked = c.chunked;
+        plan->C = c.C;
+        plan->klo = c.klo;
+        plan->split_mul = c.split;
+        if ((st = finalize_plan(plan))) break;
+        if ((st = run_dispatch(plan, false, d_coef, d_grid, s)) || (st = run_dispatch(plan, true, d_grid, d_coef, s))) break;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && st == FFTSO3_OK; r++) {
+            st = run_dispatch(plan, false, d_coef, d_grid, s);
+            if (st == FFTSO3_OK) st = run_dispatch(plan, true, d_grid, d_coef, s);
+        }
+        if (st) break;
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; win = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan->force_chunked = win.chunked;
+    plan->C = win.C;
+    plan->klo = win.klo;
+    plan->split_mul = win.split;
+    const int st2 = finalize_plan(plan);
+    plan->timing = timing;
+    plan->graph = graph;
+    return st ? st : st2;
 }
 
 int fftso3_forward(fftso3_plan_t plan, const void *d_grid, void *d_coef, void *stream)

@@ -84,6 +84,19 @@ int fftso3_plan_destroy(fftso3_plan_t plan);
 int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk);
 int fftso3_plan_get_chunk(fftso3_plan_t plan, int *chunk);
 
+/* Graph mode (on = 1): each (direction, input, output, stream) combination is captured into a
+ * hipGraph on its first call and replayed on later calls with the same arguments (at most 16 are
+ * kept; changing the plan's schedule drops them).  Calls on the NULL stream, and calls while
+ * per-stage timing is on, run eagerly.  No counterpart in the reference, which has no device
+ * path; it serves callers that repeat the same transform on the same buffers. */
+int fftso3_plan_set_graph(fftso3_plan_t plan, int on);
+
+/* Schedule tuning: times `reps` inverse + forward pairs under each candidate schedule (chunk size
+ * of the chunked kernels; row blocking of the spectrum workspace and group split of the
+ * batch-wide kernels) and keeps the fastest.  d_grid and d_coef serve as scratch: their contents
+ * are undefined afterwards.  No counterpart in the reference. */
+int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps);
+
 /* Element offsets of function i in the packed grid / coefficient arrays, and the totals. */
 long fftso3_plan_grid_offset(fftso3_plan_t plan, int fn);
 long fftso3_plan_coef_offset(fftso3_plan_t plan, int fn);
