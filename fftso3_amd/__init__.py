@@ -30,6 +30,8 @@ _SIGS = {
     "fftso3_plan_set_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_plan_get_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "fftso3_plan_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fftso3_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                       ctypes.c_void_p, ctypes.c_void_p]),
     "fftso3_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_int]),
     "fftso3_plan_grid_offset": (ctypes.c_long, [ctypes.c_void_p, ctypes.c_int]),
@@ -187,6 +189,25 @@ class Plan:
         _check(lib().fftso3_inverse(self._h, ctypes.c_void_p(coef.data_ptr()), ctypes.c_void_p(grid.data_ptr()),
                                     self._stream(stream)))
         return grid
+
+    def evaluate(self, coef, angles, fn=0, out=None, stream=None):
+        """Values of function fn at arbitrary points: angles is a float64 CUDA tensor of shape
+        [n, 3] (alpha, beta, gamma) for SO(3) or [n, 2] (theta, phi) for S^2."""
+        import torch
+        gdt, cdt = self.dtypes()
+        self._validate(coef, self.coef_elems(), cdt, "coefficients")
+        per = 2 if self.domain == "s2" else 3
+        if angles.dtype != torch.float64 or not angles.is_cuda or not angles.is_contiguous() or angles.numel() % per:
+            raise ValueError("angles must be a contiguous float64 CUDA tensor with %d values per point" % per)
+        n = angles.numel() // per
+        vdt = gdt
+        if out is None:
+            out = torch.empty(n, dtype=vdt, device=coef.device)
+        self._validate(out, n, vdt, "values")
+        _check(lib().fftso3_evaluate(self._h, int(fn), ctypes.c_void_p(coef.data_ptr()),
+                                     ctypes.c_void_p(angles.data_ptr()), n, ctypes.c_void_p(out.data_ptr()),
+                                     self._stream(stream)))
+        return out
 
     def set_graph(self, on=True):
         """Capture each (direction, input, output, stream) into a hipGraph and replay it."""

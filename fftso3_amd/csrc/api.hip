@@ -1015,6 +1015,58 @@ static int run_dispatch(fftso3_plan_t plan, bool fwd, const void *in, void *out,
     return FFTSO3_OK;
 }
 
+extern "C++" {
+template <typename T>
+static int evaluate(fftso3_plan_s *pl, int fn, const void *coef, const double *ang, long npts, void *out, hipStream_t s)
+{
+    typedef typename cplx_t<T>::type T2;
+    Class *c = nullptr;
+    int local = -1;
+    for (auto *k : pl->classes)
+        for (size_t i = 0; i < k->fns.size(); i++)
+            if (k->fns[i] == fn) { c = k; local = (int)i; }
+    if (!c) return fail(FFTSO3_EINVAL, "function %d not in the plan", fn);
+    const bool s2 = pl->domain == FFTSO3_DOMAIN_S2, real = pl->kind == FFTSO3_REAL;
+    EvalArgs<T> a;
+    a.B = c->B;
+    a.rt = rec_tables(c);
+    a.ang = ang;
+    a.npts = npts;
+    a.out = out;
+    a.real = real ? 1 : 0;
+    if (real) {
+        // complex coefficients F_c = T^H F_r T (S^2: T^T F_r) of this function, in d_fc
+        ChunkArgs ca = chunk_args<T>(pl, c, local, 1);
+        const long nc = s2 ? (long)c->B * c->B : so3_ncoef(c->B);
+        const int blocks = (int)std::min<long>(4096, (nc + 255) / 256);
+        if (s2) hipLaunchKernelGGL((s2_tmix<T, 1>), dim3(blocks), dim3(256), 0, s, ca, coef, pl->d_fc.p, nc);
+        else hipLaunchKernelGGL((so3_tmix<T, 1>), dim3(blocks), dim3(256), 0, s, ca, coef, pl->d_fc.p, nc);
+        HIP_TRY(hipGetLastError());
+        a.F = pl->d_fc.p;
+    } else {
+        a.F = reinterpret_cast<const T2 *>(coef) + pl->coef_off[fn];
+    }
+    const int grid = (int)std::min<long>(npts, 65536);
+    if (s2) hipLaunchKernelGGL((eval_points<T, 2>), dim3(grid), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((eval_points<T, 4>), dim3(grid), dim3(256), 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+}  // extern "C++"
+
+int fftso3_evaluate(fftso3_plan_t plan, int fn, const void *d_coef, const double *d_angles, long n_points,
+                    void *d_values, void *stream)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (fn < 0 || fn >= plan->n_fn) return fail(FFTSO3_EINVAL, "function index %d outside [0, %d)", fn, plan->n_fn);
+    if (n_points < 0) return fail(FFTSO3_EINVAL, "n_points must be >= 0 (got %ld)", n_points);
+    if (n_points == 0) return FFTSO3_OK;
+    if (!d_coef || !d_angles || !d_values) return fail(FFTSO3_EINVAL, "NULL device pointer");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    return plan->prec == FFTSO3_FP64 ? evaluate<double>(plan, fn, d_coef, d_angles, n_points, d_values, s)
+                                     : evaluate<float>(plan, fn, d_coef, d_angles, n_points, d_values, s);
+}
+
 int fftso3_plan_set_graph(fftso3_plan_t plan, int on)
 {
     if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");

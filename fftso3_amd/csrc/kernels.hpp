@@ -797,3 +797,103 @@ __global__ void s2_tmix(ChunkArgs ca, const void *src, void *dst, long ws_stride
         }
     }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Pointwise evaluation at arbitrary rotations (SO(3)) or directions (S^2): one workgroup per point,
+// threads over the (m,n) groups, FP64 recursion at the point's beta and FP64 accumulation
+//   SO(3): f = sum_{m,n} exp(-i(m alpha + n gamma)) sum_l (2l+1) F^l_mn d^l_mn(beta)
+//          (src/fdcl_FFTSO3.cpp:498-531)
+//   S^2:   f = sum_m exp(i m phi) sum_l F^l_m nP^l_|m|(cos theta) (-1)^m for m < 0
+//          (src/fdcl_FFTS2.cpp:194-230)
+// beta (theta) is reduced to [0, pi] first: d(beta) = d(beta + 2 pi), d(2 pi - b) = (-1)^(m-n) d(b)
+// (S^2: nP^l_m(cos(2 pi - t)) carries (-1)^m through sin^m).
+// ---------------------------------------------------------------------------------------------
+template <typename T> struct EvalArgs {
+    int B;
+    RecTables rt;
+    const void *F;        // complex coefficients of the function (packed), storage type T
+    const double *ang;    // SO(3): (alpha, beta, gamma) per point; S^2: (theta, phi)
+    long npts;
+    void *out;            // complex values, or their real parts when real != 0
+    int real;
+};
+
+template <typename T, int NP>
+__global__ void __launch_bounds__(256) eval_points(EvalArgs<T> a)
+{
+    typedef typename cplx_t<T>::type T2;
+    __shared__ double sh[3];
+    __shared__ double red[2][4];
+    const int B = a.B, ngrp = NP == 2 ? B : B * B, tid = threadIdx.x;
+    const T2 *F = reinterpret_cast<const T2 *>(a.F);
+    for (long pt = blockIdx.x; pt < a.npts; pt += gridDim.x) {
+        double al, be, ga;
+        if (NP == 2) { be = a.ang[2 * pt]; al = a.ang[2 * pt + 1]; ga = 0.0; }
+        else { al = a.ang[3 * pt]; be = a.ang[3 * pt + 1]; ga = a.ang[3 * pt + 2]; }
+        const double tp = 2.0 * M_PI;
+        be = fmod(be, tp);
+        if (be < 0) be += tp;
+        const bool fold = be > M_PI;
+        if (fold) be = tp - be;
+        if (tid == 0) {
+            sh[0] = cos(be);
+            sh[1] = log(cos(0.5 * be));
+            sh[2] = NP == 2 ? log(sin(be)) : log(sin(0.5 * be));  // the tables' S^2 seed uses sin(theta)
+        }
+        __syncthreads();
+        RecTables t = a.rt;
+        t.cb = sh;
+        t.lnc = sh + 1;
+        t.lns = sh + 2;
+        double re = 0.0, im = 0.0;
+        for (int g = tid; g < ngrp; g += 256) {
+            int m0, n0;
+            if (NP == 2) { m0 = g; n0 = 0; } else so3_group(g, m0, n0);
+            double ar[NP], ai[NP];
+#pragma unroll
+            for (int p = 0; p < NP; p++) { ar[p] = 0.0; ai[p] = 0.0; }
+            int pm[NP], pn[NP];
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                if (NP == 2) { pm[p] = p ? -m0 : m0; pn[p] = 0; }
+                else so3_pair(m0, n0, p, pm[p], pn[p]);
+            }
+            group_recursion(t, NP == 2, B, g, m0, n0, 0, B, [&](int l, double v) {
+                const double c = NP == 2 ? v : (double)(2 * l + 1) * v;
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const T2 f = F[NP == 2 ? s2_idx(l, pm[p]) : so3_idx(l, pm[p], pn[p])];
+                    ar[p] += c * (double)f.x;
+                    ai[p] += c * (double)f.y;
+                }
+            });
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const bool valid = NP == 2 ? (p == 0 || m0 > 0) : so3_pair_valid(m0, n0, p);
+                if (!valid) continue;
+                double sgn = NP == 2 ? (p ? (double)parity_sign(m0) : 1.0) : (double)so3_pair_sign(m0, n0, p);
+                if (fold) sgn *= (double)parity_sign(NP == 2 ? pm[p] : pm[p] - pn[p]);
+                // SO(3): exp(-i(m alpha + n gamma)); S^2: exp(+i m phi)
+                const double ph = NP == 2 ? (double)pm[p] * al : -((double)pm[p] * al + (double)pn[p] * ga);
+                double sp, cp;
+                sincos(ph, &sp, &cp);
+                re += sgn * (ar[p] * cp - ai[p] * sp);
+                im += sgn * (ar[p] * sp + ai[p] * cp);
+            }
+        }
+        // workgroup sum: waves by shuffles, then across the 4 waves through LDS
+        for (int o = 32; o > 0; o >>= 1) {
+            re += __shfl_down(re, o);
+            im += __shfl_down(im, o);
+        }
+        if ((tid & 63) == 0) { red[0][tid >> 6] = re; red[1][tid >> 6] = im; }
+        __syncthreads();
+        if (tid == 0) {
+            const double r = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+            const double i = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+            if (a.real) reinterpret_cast<T *>(a.out)[pt] = (T)r;
+            else reinterpret_cast<T2 *>(a.out)[pt] = cmk<T2>((T)r, (T)i);
+        }
+        __syncthreads();
+    }
+}

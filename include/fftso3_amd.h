@@ -112,6 +112,17 @@ int fftso3_forward(fftso3_plan_t plan, const void *d_grid, void *d_coef, void *s
 /* coefficients -> grid (inverse_transform at every grid point) */
 int fftso3_inverse(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream);
 
+/* Pointwise inverse of function fn of the plan's batch at n_points arbitrary points
+ * (inverse_transform(F, alpha, beta, gamma), src/fdcl_FFTSO3.cpp:498-531 and 1141-1167;
+ * inverse_transform(F, theta, phi), src/fdcl_FFTS2.cpp:194-230 and 500-512), batched:
+ *   d_coef    the plan's whole coefficient array (as for fftso3_inverse)
+ *   d_angles  doubles: (alpha, beta, gamma) per point for SO(3), (theta, phi) for S^2
+ *   d_values  one value per point: complex for FFTSO3_COMPLEX plans, real for FFTSO3_REAL plans,
+ *             in the plan's storage type.
+ * The recursion and the sums run in FP64 at every precision. */
+int fftso3_evaluate(fftso3_plan_t plan, int fn, const void *d_coef, const double *d_angles, long n_points,
+                    void *d_values, void *stream);
+
 /* --- diagnostics ----------------------------------------------------------------------------- */
 
 const char *fftso3_status_string(int status);
