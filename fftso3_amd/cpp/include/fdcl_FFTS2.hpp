@@ -42,6 +42,9 @@ public:
     complex<double> inverse_transform(FFTS2_matrix_complex F, double theta, double phi);
     std::vector<complex<double>> inverse_transform(const FFTS2_matrix_complex &F);
     std::vector<complex<double>> inverse_transform_batch(const std::vector<FFTS2_matrix_complex> &F);
+    // pointwise inverse at many directions on the GPU: angles holds (theta, phi) per point
+    std::vector<complex<double>> inverse_transform_points(const FFTS2_matrix_complex &F,
+                                                          const std::vector<double> &angles);
 
     void check_all();
     double check_weight();
@@ -80,6 +83,7 @@ public:
     double inverse_transform(FFTS2_matrix_real F, double theta, double phi);
     std::vector<double> inverse_transform(const FFTS2_matrix_real &F);
     std::vector<double> inverse_transform_batch(const std::vector<FFTS2_matrix_real> &F);
+    std::vector<double> inverse_transform_points(const FFTS2_matrix_real &F, const std::vector<double> &angles);
 
     void check_all();
     double check_transform();

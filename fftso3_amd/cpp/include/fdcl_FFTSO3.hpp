@@ -27,6 +27,14 @@
 #include "fdcl_FFTSO3_matrix.hpp"
 #include "fftso3_amd.h"
 
+// Callers that have Eigen get the reference's Eigen::Matrix3d overloads as inline adapters.
+#if defined(__has_include)
+#if __has_include(<Eigen/Dense>)
+#include <Eigen/Dense>
+#define FDCL_HAVE_EIGEN 1
+#endif
+#endif
+
 namespace fdcl
 {
 
@@ -36,6 +44,23 @@ struct Matrix3d {
     double &operator()(int i, int j) { return a[3 * i + j]; }
     double operator()(int i, int j) const { return a[3 * i + j]; }
 };
+
+#ifdef FDCL_HAVE_EIGEN
+inline Matrix3d to_matrix3d(const Eigen::Matrix3d &E)
+{
+    Matrix3d R;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) R(i, j) = E(i, j);
+    return R;
+}
+inline Eigen::Matrix3d to_eigen(const Matrix3d &R)
+{
+    Eigen::Matrix3d E;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) E(i, j) = R(i, j);
+    return E;
+}
+#endif
 
 // 3-2-3 Euler angles <-> rotation matrix (src/misc_matrix_func.cpp:144-178)
 std::vector<double> R2Euler323(const Matrix3d &R);
@@ -52,6 +77,9 @@ public:
     // grid / coefficient arrays of n_fn functions, host side in double or complex<double>
     void forward(int domain, int l_max, int n_fn, int kind, int precision, const void *grid, void *coef);
     void inverse(int domain, int l_max, int n_fn, int kind, int precision, const void *coef, void *grid);
+    // values of one function at n points (angles: 3 or 2 doubles per point), host arrays
+    void evaluate(int domain, int l_max, int kind, int precision, const void *coef, const double *angles, long n,
+                  void *values);
 
 private:
     void prepare(int domain, int l_max, int n_fn, int kind, int precision);
@@ -60,6 +88,8 @@ private:
     void *d_grid_ = nullptr, *d_coef_ = nullptr;
     long grid_elems_ = 0, coef_elems_ = 0;
     std::vector<char> stage_;  // host staging for the single-precision plans
+    void *d_ang_ = nullptr, *d_val_ = nullptr;
+    long pts_cap_ = 0;
 };
 }  // namespace detail
 
@@ -99,6 +129,9 @@ public:
     // inverse on the whole grid [k][j1][j2] (on the GPU)
     std::vector<complex<double>> inverse_transform(const FFTSO3_matrix_complex &F);
     std::vector<complex<double>> inverse_transform_batch(const std::vector<FFTSO3_matrix_complex> &F);
+    // pointwise inverse at many rotations on the GPU: angles holds (alpha, beta, gamma) per point
+    std::vector<complex<double>> inverse_transform_points(const FFTSO3_matrix_complex &F,
+                                                          const std::vector<double> &angles);
 
     // character chi_l(theta) = sin((2l+1) theta / 2) / sin(theta / 2) (src/fdcl_FFTSO3.cpp:404-415)
     std::vector<double> character(double theta);
@@ -109,6 +142,18 @@ public:
     double check_weight();
     double check_wigner_d();
     double check_transform();
+
+#ifdef FDCL_HAVE_EIGEN
+    FFTSO3_matrix_complex wigner_D(const Eigen::Matrix3d &R) { return wigner_D(to_matrix3d(R)); }
+    FFTSO3_matrix_complex forward_transform(std::function<complex<double>(Eigen::Matrix3d)> func)
+    {
+        return forward_transform(std::function<complex<double>(Matrix3d)>([&](Matrix3d R) { return func(to_eigen(R)); }));
+    }
+    complex<double> inverse_transform(FFTSO3_matrix_complex F, const Eigen::Matrix3d &R)
+    {
+        return inverse_transform(std::move(F), to_matrix3d(R));
+    }
+#endif
 
     // grid nodes
     double alpha_j(int j) const;
@@ -147,6 +192,19 @@ public:
     double inverse_transform(FFTSO3_matrix_real F, const Matrix3d &R);
     std::vector<double> inverse_transform(const FFTSO3_matrix_real &F);
     std::vector<double> inverse_transform_batch(const std::vector<FFTSO3_matrix_real> &F);
+    std::vector<double> inverse_transform_points(const FFTSO3_matrix_real &F, const std::vector<double> &angles);
+
+#ifdef FDCL_HAVE_EIGEN
+    FFTSO3_matrix_real real_harmonics(const Eigen::Matrix3d &R) { return real_harmonics(to_matrix3d(R)); }
+    FFTSO3_matrix_real forward_transform(std::function<double(Eigen::Matrix3d)> func)
+    {
+        return forward_transform(std::function<double(Matrix3d)>([&](Matrix3d R) { return func(to_eigen(R)); }));
+    }
+    double inverse_transform(FFTSO3_matrix_real F, const Eigen::Matrix3d &R)
+    {
+        return inverse_transform(std::move(F), to_matrix3d(R));
+    }
+#endif
 
     // T^l of src/fdcl_FFTSO3.cpp:1107-1139
     FFTSO3_matrix_complex matrix2rsph(int L);

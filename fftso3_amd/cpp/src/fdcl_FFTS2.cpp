@@ -134,6 +134,17 @@ std::vector<complex<double>> FFTS2_complex::inverse_transform_batch(const std::v
     return grid;
 }
 
+std::vector<complex<double>> FFTS2_complex::inverse_transform_points(const FFTS2_matrix_complex &F,
+                                                                     const std::vector<double> &angles)
+{
+    if (angles.size() % 2) throw std::invalid_argument("inverse_transform_points: 2 angles per point");
+    init(F.l_max);
+    const long n = (long)angles.size() / 2;
+    std::vector<complex<double>> v(n);
+    dev_->evaluate(FFTSO3_DOMAIN_S2, l_max, FFTSO3_COMPLEX, precision_, F.data(), angles.data(), n, v.data());
+    return v;
+}
+
 // sum_k w_k Y^l_0(theta_k, 0) B / sqrt(pi) = delta_{l0} for l < 2B (src/fdcl_FFTS2.cpp:243-262)
 double FFTS2_complex::check_weight()
 {
@@ -249,6 +260,16 @@ std::vector<double> FFTS2_real::inverse_transform_batch(const std::vector<FFTS2_
     std::vector<double> grid(N * N * F.size());
     dev_->inverse(FFTSO3_DOMAIN_S2, l_max, (int)F.size(), FFTSO3_REAL, precision_, coef.data(), grid.data());
     return grid;
+}
+
+std::vector<double> FFTS2_real::inverse_transform_points(const FFTS2_matrix_real &F, const std::vector<double> &angles)
+{
+    if (angles.size() % 2) throw std::invalid_argument("inverse_transform_points: 2 angles per point");
+    init(F.l_max);
+    const long n = (long)angles.size() / 2;
+    std::vector<double> v(n);
+    dev_->evaluate(FFTSO3_DOMAIN_S2, l_max, FFTSO3_REAL, precision_, F.data(), angles.data(), n, v.data());
+    return v;
 }
 
 double FFTS2_real::check_transform()

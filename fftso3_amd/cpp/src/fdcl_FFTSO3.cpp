@@ -218,6 +218,17 @@ std::vector<complex<double>> FFTSO3_complex::inverse_transform_batch(const std::
     return grid;
 }
 
+std::vector<complex<double>> FFTSO3_complex::inverse_transform_points(const FFTSO3_matrix_complex &F,
+                                                                      const std::vector<double> &angles)
+{
+    if (angles.size() % 3) throw std::invalid_argument("inverse_transform_points: 3 angles per point");
+    init(F.l_max);
+    const long n = (long)angles.size() / 3;
+    std::vector<complex<double>> v(n);
+    dev_->evaluate(FFTSO3_DOMAIN_SO3, l_max, FFTSO3_COMPLEX, precision_, F.data(), angles.data(), n, v.data());
+    return v;
+}
+
 // sum_k w_k d^l_00(beta_k) 4B^2 = delta_{l0} for l < 2B (src/fdcl_FFTSO3.cpp:258-288)
 double FFTSO3_complex::check_weight()
 {
@@ -417,6 +428,16 @@ std::vector<double> FFTSO3_real::inverse_transform_batch(const std::vector<FFTSO
     std::vector<double> grid(N * N * N * F.size());
     dev_->inverse(FFTSO3_DOMAIN_SO3, l_max, (int)F.size(), FFTSO3_REAL, precision_, coef.data(), grid.data());
     return grid;
+}
+
+std::vector<double> FFTSO3_real::inverse_transform_points(const FFTSO3_matrix_real &F, const std::vector<double> &angles)
+{
+    if (angles.size() % 3) throw std::invalid_argument("inverse_transform_points: 3 angles per point");
+    init(F.l_max);
+    const long n = (long)angles.size() / 3;
+    std::vector<double> v(n);
+    dev_->evaluate(FFTSO3_DOMAIN_SO3, l_max, FFTSO3_REAL, precision_, F.data(), angles.data(), n, v.data());
+    return v;
 }
 
 // the real harmonics are real, orthogonal (U U^T = I) and agree with the explicit d-based form of

@@ -65,6 +65,8 @@ DevicePlan::~DevicePlan()
     if (plan_) fftso3_plan_destroy(plan_);
     if (d_grid_) (void)hipFree(d_grid_);
     if (d_coef_) (void)hipFree(d_coef_);
+    if (d_ang_) (void)hipFree(d_ang_);
+    if (d_val_) (void)hipFree(d_val_);
 }
 
 void DevicePlan::prepare(int domain, int l_max, int n_fn, int kind, int precision)
@@ -132,6 +134,29 @@ void DevicePlan::inverse(int domain, int l_max, int n_fn, int kind, int precisio
     check_abi(fftso3_inverse(plan_, d_coef_, d_grid_, nullptr), "fftso3_inverse");
     check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
     download(d_grid_, grid_elems_ * w, static_cast<double *>(grid), precision, stage_);
+}
+
+void DevicePlan::evaluate(int domain, int l_max, int kind, int precision, const void *coef, const double *angles,
+                          long n, void *values)
+{
+    prepare(domain, l_max, 1, kind, precision);
+    if (n <= 0) return;
+    const int per = domain == FFTSO3_DOMAIN_S2 ? 2 : 3;
+    if (n > pts_cap_) {
+        if (d_ang_) (void)hipFree(d_ang_);
+        if (d_val_) (void)hipFree(d_val_);
+        d_ang_ = d_val_ = nullptr;
+        check_hip(hipMalloc(&d_ang_, n * per * sizeof(double)), "hipMalloc(angles)");
+        check_hip(hipMalloc(&d_val_, n * 2 * sizeof(double)), "hipMalloc(values)");
+        pts_cap_ = n;
+    }
+    const int w = kind == FFTSO3_COMPLEX ? 2 : 1;
+    upload(static_cast<const double *>(coef), coef_elems_ * w, d_coef_, precision, stage_);
+    check_hip(hipMemcpy(d_ang_, angles, n * per * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    check_abi(fftso3_evaluate(plan_, 0, d_coef_, static_cast<const double *>(d_ang_), n, d_val_, nullptr),
+              "fftso3_evaluate");
+    check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    download(d_val_, n * w, static_cast<double *>(values), precision, stage_);
 }
 
 }  // namespace detail

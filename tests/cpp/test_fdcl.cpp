@@ -164,6 +164,46 @@ static int gpu_checks()
         const auto F3 = fc.forward_transform(fc.inverse_transform(F));
         expect("so3_fp32_round_trip", rel(F3.v, F.v), 2e-6);
     }
+    // batched pointwise evaluation on the GPU against the host pointwise inverse
+    {
+        const int L = 10;
+        fdcl::FFTSO3_complex fc(L);
+        fdcl::FFTSO3_matrix_complex F(L);
+        F.setRandom();
+        std::vector<double> ang;
+        for (int s = 0; s < 25; s++) {
+            ang.push_back(0.37 * s - 2.0);
+            ang.push_back(0.13 * s);
+            ang.push_back(1.0 - 0.29 * s);
+        }
+        const auto v = fc.inverse_transform_points(F, ang);
+        std::vector<fdcl::complex<double>> ref;
+        for (int s = 0; s < 25; s++) ref.push_back(fc.inverse_transform(F, ang[3 * s], ang[3 * s + 1], ang[3 * s + 2]));
+        expect("so3_points_vs_pointwise", rel(v, ref), 1e-12);
+        fdcl::FFTSO3_real fr(L);
+        fdcl::FFTSO3_matrix_real Fr(L);
+        Fr.setRandom();
+        const auto vr = fr.inverse_transform_points(Fr, ang);
+        double e = 0, nn = 0;
+        for (int s = 0; s < 25; s++) {
+            const double r = fr.inverse_transform(Fr, ang[3 * s], ang[3 * s + 1], ang[3 * s + 2]);
+            e += (vr[s] - r) * (vr[s] - r);
+            nn += r * r;
+        }
+        expect("so3_real_points_vs_pointwise", std::sqrt(e / nn), 1e-12);
+        fdcl::FFTS2_complex sc(L);
+        fdcl::FFTS2_matrix_complex G(L);
+        G.setRandom();
+        std::vector<double> a2;
+        for (int s = 0; s < 25; s++) {
+            a2.push_back(0.12 * s);
+            a2.push_back(0.4 * s - 3.0);
+        }
+        const auto vs = sc.inverse_transform_points(G, a2);
+        std::vector<fdcl::complex<double>> rs;
+        for (int s = 0; s < 25; s++) rs.push_back(sc.inverse_transform(G, a2[2 * s], a2[2 * s + 1]));
+        expect("s2_points_vs_pointwise", rel(vs, rs), 1e-12);
+    }
     {
         const int L = 20, N = 2 * (L + 1);
         fdcl::FFTS2_complex sc(L);
