@@ -88,6 +88,8 @@ struct Class {
     int row_tile = 0, col_tile = 0; // two-pass fallback tiles
     bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
     bool mxk = false;               // ... with the K-chunked kernels (128 < N, B <= 128)
+    bool pt = false;                // chunked contraction reads / writes transposed coefficients PT
+    DevBuf d_dtab, d_dtab_off;      // precomputed d table of the chunked path at large B (or empty)
     int chunk = 8;                  // functions per pipeline pass for this class
     DevBuf d_items_mx;              // groups of the batch-wide path (largest first)
     int nitems_mx = 0;
@@ -268,6 +270,20 @@ int build_class_tables(fftso3_plan_s *pl, Class *c)
     return FFTSO3_OK;
 }
 
+RecTables rec_tables(Class *c)
+{
+    RecTables rt;
+    rt.cb = (const double *)c->d_cb.p;
+    rt.lnc = (const double *)c->d_lnc.p;
+    rt.lns = (const double *)c->d_lns.p;
+    rt.w = (const double *)c->d_w.p;
+    rt.lnbin = (const double *)c->d_lnbin.p;
+    rt.sq = (const double *)c->d_sq.p;
+    rt.isq = (const double *)c->d_isq.p;
+    rt.lcoef = (const double *)c->d_lcoef.p;
+    return rt;
+}
+
 int build_class_work(fftso3_plan_s *pl, Class *c)
 {
     const int B = c->B, N = c->N;
@@ -281,6 +297,41 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
         c->mx = (pl->prec == FFTSO3_FP32 && N <= 128 && !pl->force_chunked) || c->mxk;
     }
     c->chunk = c->mx ? (int)c->fns.size() : pl->C;
+    // large bandlimits on the chunked kernels: coefficients go through PT[e][fi] so the contraction
+    // moves whole rows of the chunk's functions instead of scattered single values
+    c->pt = !c->mx && pl->kind == FFTSO3_COMPLEX && B > 64 && pl->prec != FFTSO3_BF16X3 &&
+            pl->prec != FFTSO3_BF16X6;
+    // Above l_max = 127 the chunked contraction would spend its time in the sequential recursion
+    // (one lane per k, the rest of the workgroup idle): the plan computes d once, at full
+    // occupancy, into a table it keeps ([g][l - m0][k] in the storage type; 11.5 GB at l_max = 255
+    // in FP32), if that stays within 64 GB.
+    if (!c->mx && B > 128 && pl->prec != FFTSO3_BF16X3 && pl->prec != FFTSO3_BF16X6) {
+        const bool s2d = pl->domain == FFTSO3_DOMAIN_S2;
+        const int ngrp = s2d ? B : B * B;
+        const size_t tsz = pl->esz / 2;
+        std::vector<long> goff(ngrp);
+        long rows = 0;
+        for (int g = 0; g < ngrp; g++) {
+            int m0 = s2d ? g : (int)std::sqrt((double)g + 0.5);
+            if (!s2d) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
+            goff[g] = rows * N;
+            rows += B - m0;
+        }
+        const size_t bytes = (size_t)rows * N * tsz;
+        if (bytes <= ((size_t)64 << 30)) {
+            int st;
+            if ((st = c->d_dtab_off.upload(goff)) || (st = c->d_dtab.alloc(bytes))) return st;
+            const long total = (long)ngrp * N;
+            const int blocks = (int)((total + 255) / 256);
+            RecTables rt = rec_tables(c);
+            if (tsz == 8) hipLaunchKernelGGL((build_dtab<double>), dim3(blocks), dim3(256), 0, 0, rt, B, N, s2d ? 1 : 0,
+                                             (const long *)c->d_dtab_off.p, (double *)c->d_dtab.p);
+            else hipLaunchKernelGGL((build_dtab<float>), dim3(blocks), dim3(256), 0, 0, rt, B, N, s2d ? 1 : 0,
+                                    (const long *)c->d_dtab_off.p, (float *)c->d_dtab.p);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipDeviceSynchronize());
+        }
+    }
     const int C = c->mx ? 8 : c->chunk;  // tiling of the chunked kernels (unused by the MX path)
     const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
     const size_t tsz = pl->esz / 2;  // real element size
@@ -385,7 +436,7 @@ int finalize_plan(fftso3_plan_s *pl)
         if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
         if (pl->kind == FFTSO3_REAL)
             fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
-        if (c->mx) {
+        if (c->mx || c->pt) {
             fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);  // PT
             if (pl->kind == FFTSO3_REAL) tmpb = std::max(tmpb, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
         }
@@ -482,19 +533,6 @@ FftArgs<T> fft_args(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, const void
     return a;
 }
 
-RecTables rec_tables(Class *c)
-{
-    RecTables rt;
-    rt.cb = (const double *)c->d_cb.p;
-    rt.lnc = (const double *)c->d_lnc.p;
-    rt.lns = (const double *)c->d_lns.p;
-    rt.w = (const double *)c->d_w.p;
-    rt.lnbin = (const double *)c->d_lnbin.p;
-    rt.sq = (const double *)c->d_sq.p;
-    rt.isq = (const double *)c->d_isq.p;
-    rt.lcoef = (const double *)c->d_lcoef.p;
-    return rt;
-}
 
 template <typename T, int C>
 int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst,
@@ -504,6 +542,9 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
     a.ca = ca;
     a.rt = rec_tables(c);
     a.s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    a.pt = c->pt ? 1 : 0;
+    a.dtab = c->d_dtab.p;
+    a.dtab_off = (const long *)c->d_dtab_off.p;
     a.src = src;
     a.dst = dst;
     const int np = a.s2 ? 2 : 4, ncol = 2 * np * C;
@@ -587,7 +628,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             HIP_TRY(hipGetLastError());
         }
         StageTimer t(pl, s, 2);
-        hipLaunchKernelGGL((coef_transpose<0>), dim3(tgrid), dim3(256), 0, s, ct, ncp, pl->d_fc.p,
+        hipLaunchKernelGGL((coef_transpose<0, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, pl->d_fc.p,
                            real ? pl->d_tmp.p : out);
         HIP_TRY(hipGetLastError());
         if (real) {
@@ -603,7 +644,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
                 else hipLaunchKernelGGL((so3_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
                 HIP_TRY(hipGetLastError());
             }
-            hipLaunchKernelGGL((coef_transpose<1>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
+            hipLaunchKernelGGL((coef_transpose<1, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
                                pl->d_fc.p);
             HIP_TRY(hipGetLastError());
         }
@@ -628,6 +669,9 @@ int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const v
             a.ca = ca;
             a.rt = rec_tables(c);
             a.s2 = pl->domain == FFTSO3_DOMAIN_S2;
+            a.pt = 0;
+            a.dtab = nullptr;
+            a.dtab_off = nullptr;
             a.src = src;
             a.dst = dst;
             const bool x6 = pl->prec == FFTSO3_BF16X6;
@@ -704,7 +748,13 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         }
         {
             StageTimer t(pl, s, 1);
-            if ((st = contract<T>(pl, c, cc, true, pl->d_ws.p, real ? pl->d_fc.p : out, s))) return st;
+            if ((st = contract<T>(pl, c, cc, true, pl->d_ws.p, (real || c->pt) ? pl->d_fc.p : out, s))) return st;
+        }
+        if (c->pt) {
+            StageTimer t(pl, s, 2);
+            const int tgrid = xcd_grid((int)(((nc + 63) / 64) * ((nfc + 31) / 32)));
+            hipLaunchKernelGGL((coef_transpose<0, T2>), dim3(tgrid), dim3(256), 0, s, ca, (int)nc, pl->d_fc.p, out);
+            HIP_TRY(hipGetLastError());
         }
         if (real) {
             StageTimer t(pl, s, 2);
@@ -725,6 +775,13 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             StageTimer t(pl, s, 3);
             if (s2) hipLaunchKernelGGL((s2_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
             else hipLaunchKernelGGL((so3_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
+            HIP_TRY(hipGetLastError());
+            csrc = pl->d_fc.p;
+        }
+        if (!contracted && c->pt) {
+            StageTimer t(pl, s, 3);
+            const int tgrid = xcd_grid((int)(((nc + 63) / 64) * ((nfc + 31) / 32)));
+            hipLaunchKernelGGL((coef_transpose<1, T2>), dim3(tgrid), dim3(256), 0, s, ca, (int)nc, in, pl->d_fc.p);
             HIP_TRY(hipGetLastError());
             csrc = pl->d_fc.p;
         }

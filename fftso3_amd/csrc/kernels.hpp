@@ -469,7 +469,25 @@ template <typename T> struct ConArgs {
     const void *src;      // forward: workspace S; inverse: coefficients
     void *dst;            // forward: coefficients; inverse: workspace G
     int s2;               // 1 for S^2 (two pairs, Legendre recursion)
+    int pt;               // coefficients in PT[e][fi] (rows of C complex values) instead of packed
+    const void *dtab;     // precomputed d^l_{m0 n0}(beta_k) per group ([l - m0][k], storage type), or null
+    const long *dtab_off; // first element of each group's block in dtab
 };
+
+// Table of d^l_{m0 n0}(beta_k) (S^2: nP^l_{m0}(cos theta_k)) for every group, k and l >= m0, laid out
+// [g][l - m0][k]: one thread per (g, k) runs the recursion; consecutive threads write consecutive k.
+template <typename T>
+__global__ void build_dtab(RecTables rt, int B, int N, int s2, const long *goff, T *tab)
+{
+    const int ngrp = s2 ? B : B * B;
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)ngrp * N) return;
+    const int g = (int)(e / N), k = (int)(e - (long)g * N);
+    int m0, n0;
+    if (s2) { m0 = g; n0 = 0; } else so3_group(g, m0, n0);
+    T *row = tab + goff[g] + k;
+    group_recursion(rt, s2, B, g, m0, n0, k, B, [&](int l, double v) { row[(long)(l - m0) * N] = (T)v; });
+}
 
 // 16-byte native vectors (clang ext_vector_type; HIP's float4 class copies lower to memcpy and
 // would keep register arrays of them in scratch)
@@ -545,14 +563,23 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
         }                                                                    \
     }
     CONTRACT_FWD_LOAD(0)
-    // FP64 recursion, one lane per k; D[k][l-l0] = w_k d^l(beta_k)
-    for (int k = tid; k < N; k += NT) {
-        const double wk = a.rt.w[k];
-        T *Dk = D + (size_t)k * LT;
-        group_recursion(a.rt, a.s2, B, g, m0, n0, k, lend, [&](int l, double v) {
-            if (l >= l0) Dk[l - l0] = (T)(wk * v);
-        });
-        for (int l = max(lend, l0); l < l0 + LT; l++) Dk[l - l0] = (T)0;
+    if (a.dtab) {
+        // D[k][l-l0] = w_k d^l(beta_k) from the precomputed table (reads contiguous in k)
+        const T *tab = reinterpret_cast<const T *>(a.dtab) + a.dtab_off[g];
+        for (int e = tid; e < LT * N; e += NT) {
+            const int lr = e / N, k = e - lr * N, l = l0 + lr;
+            D[(size_t)k * LT + lr] = l < lend ? (T)(a.rt.w[k] * (double)tab[(long)(l - m0) * N + k]) : (T)0;
+        }
+    } else {
+        // FP64 recursion, one lane per k; D[k][l-l0] = w_k d^l(beta_k)
+        for (int k = tid; k < N; k += NT) {
+            const double wk = a.rt.w[k];
+            T *Dk = D + (size_t)k * LT;
+            group_recursion(a.rt, a.s2, B, g, m0, n0, k, lend, [&](int l, double v) {
+                if (l >= l0) Dk[l - l0] = (T)(wk * v);
+            });
+            for (int l = max(lend, l0); l < l0 + LT; l++) Dk[l - l0] = (T)0;
+        }
     }
     const int rows_per_thread = 4;
     const int ngroups = NT / ncol;
@@ -597,13 +624,14 @@ __global__ void __launch_bounds__(32 * C) contract_fwd(ConArgs<T> a)
         valid = so3_pair_valid(m0, n0, p);
     }
     if (!valid) return;
-    T *out = reinterpret_cast<T *>(a.dst) + 2 * ca.coef_off[ca.fn[fi]];
+    T *out = reinterpret_cast<T *>(a.dst) + (a.pt ? 2 * fi : 2 * ca.coef_off[ca.fn[fi]]);
+    const long es = a.pt ? 2L * C : 2L;     // PT rows hold the chunk's C functions
 #pragma unroll
     for (int q = 0; q < rows_per_thread; q++) {
         int l = l0 + rg * rows_per_thread + q;
         if (l < lend) {
             long idx = a.s2 ? s2_idx(l, m) : so3_idx(l, m, n);
-            out[2 * idx + ri] = (T)sp * acc[q];
+            out[es * idx + ri] = (T)sp * acc[q];
         }
     }
 }
@@ -647,8 +675,9 @@ __global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
                     int m, n, sp;
                     if (a.s2) { m = p ? -m0 : m0; n = 0; sp = p ? parity_sign(m0) : 1; }
                     else { so3_pair(m0, n0, p, m, n); sp = so3_pair_sign(m0, n0, p); }
-                    const T2 *F = reinterpret_cast<const T2 *>(a.src) + ca.coef_off[ca.fn[fi]];
-                    T2 f = F[a.s2 ? s2_idx(l, m) : so3_idx(l, m, n)];
+                    const long idx = a.s2 ? s2_idx(l, m) : so3_idx(l, m, n);
+                    const T2 f = a.pt ? reinterpret_cast<const T2 *>(a.src)[idx * C + fi]
+                                      : reinterpret_cast<const T2 *>(a.src)[ca.coef_off[ca.fn[fi]] + idx];
                     val = cmk<T2>((T)sp * f.x, (T)sp * f.y);
                 }
                 r[i] = val;
@@ -663,14 +692,23 @@ __global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
         }
     };
     gather(0);
-    for (int kk = tid; kk < KT; kk += NT) {
-        if (kk >= kn) {
-            for (int l = m0; l < B; l++) D[(size_t)(l - m0) * KT + kk] = (T)0;
-            continue;
+    if (a.dtab) {
+        const T *tab = reinterpret_cast<const T *>(a.dtab) + a.dtab_off[g];
+        for (int e = tid; e < M * KT; e += NT) {
+            const int lr = e / KT, kk = e - lr * KT;
+            const double cl = a.s2 ? 1.0 : (double)(2 * (m0 + lr) + 1);
+            D[(size_t)lr * KT + kk] = kk < kn ? (T)(cl * (double)tab[(long)lr * N + k0 + kk]) : (T)0;
         }
-        group_recursion(a.rt, a.s2, B, g, m0, n0, k0 + kk, B, [&](int l, double v) {
-            D[(size_t)(l - m0) * KT + kk] = (T)(a.s2 ? v : (double)(2 * l + 1) * v);
-        });
+    } else {
+        for (int kk = tid; kk < KT; kk += NT) {
+            if (kk >= kn) {
+                for (int l = m0; l < B; l++) D[(size_t)(l - m0) * KT + kk] = (T)0;
+                continue;
+            }
+            group_recursion(a.rt, a.s2, B, g, m0, n0, k0 + kk, B, [&](int l, double v) {
+                D[(size_t)(l - m0) * KT + kk] = (T)(a.s2 ? v : (double)(2 * l + 1) * v);
+            });
+        }
     }
     deposit(0);
     for (int base = PF * NT; base < total; base += PF * NT) {

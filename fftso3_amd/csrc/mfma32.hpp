@@ -371,10 +371,10 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
 // indices x 32 functions go through LDS so both sides move in contiguous 512- and 256-byte runs.
 // DIR 0: PT -> packed, DIR 1: packed -> PT.
 // ---------------------------------------------------------------------------------------------
-template <int DIR>
+template <int DIR, typename T2 = float2>
 __global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, const void *src, void *dst)
 {
-    __shared__ float2 tile[32][65];
+    __shared__ T2 tile[32][65];
     const int nte = (ncoef + 63) / 64, ntf = (ca.nfc + 31) / 32, nunit = nte * ntf;
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
@@ -386,28 +386,28 @@ __global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, c
     const int fi = f0 + fl;
     if (DIR == 1) {
         if (fi < ca.nfc) {
-            const float2 *P = reinterpret_cast<const float2 *>(src) + ca.coef_off[ca.fn[fi]] + e0 + eo;
+            const T2 *P = reinterpret_cast<const T2 *>(src) + ca.coef_off[ca.fn[fi]] + e0 + eo;
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 if (e0 + eo + j < ncoef) tile[fl][eo + j] = P[j];
         }
         __syncthreads();
         if (e0 + el < ncoef) {
-            float2 *Q = reinterpret_cast<float2 *>(dst) + (long)(e0 + el) * ca.C + f0 + fo;
+            T2 *Q = reinterpret_cast<T2 *>(dst) + (long)(e0 + el) * ca.C + f0 + fo;
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 if (f0 + fo + j < ca.nfc) Q[j] = tile[fo + j][el];
         }
     } else {
         if (e0 + el < ncoef) {
-            const float2 *Q = reinterpret_cast<const float2 *>(src) + (long)(e0 + el) * ca.C + f0 + fo;
+            const T2 *Q = reinterpret_cast<const T2 *>(src) + (long)(e0 + el) * ca.C + f0 + fo;
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 if (f0 + fo + j < ca.nfc) tile[fo + j][el] = Q[j];
         }
         __syncthreads();
         if (fi < ca.nfc) {
-            float2 *P = reinterpret_cast<float2 *>(dst) + ca.coef_off[ca.fn[fi]] + e0 + eo;
+            T2 *P = reinterpret_cast<T2 *>(dst) + ca.coef_off[ca.fn[fi]] + e0 + eo;
 #pragma unroll
             for (int j = 0; j < 8; j++)
                 if (e0 + eo + j < ncoef) P[j] = tile[fl][eo + j];

@@ -49,9 +49,10 @@ def test_so3_complex_fp64_b16(fftso3, torch_cuda, oracle):
     assert rel_l2(c[0], F[0]) < FP64_TOL
 
 
-@pytest.mark.parametrize("B", [1, 2, 3, 6, 11, 21, 24, 32])
+@pytest.mark.parametrize("B", [1, 2, 3, 6, 11, 21, 24, 32, 80])
 def test_so3_complex_fp64_sizes(fftso3, torch_cuda, oracle, B):
-    """Bandlimits whose 2B is 2, 4, 6, 12, 22 (direct DFT), 42, 48 and 64."""
+    """Bandlimits whose 2B is 2, 4, 6, 12, 22 (direct DFT), 42, 48, 64 and 160 (transposed
+    coefficient staging of the chunked kernels above l_max = 63)."""
     F = inputs.so3_coefficients(B, 1, 100 + B)
     g_ref = oracle.so3_inverse(B, F[0])
     g, c = so3_roundtrip(fftso3, torch_cuda, B, F, grid_in=g_ref[None])
@@ -137,7 +138,7 @@ def s2_roundtrip(fftso3, torch, B, F, kind="complex", prec="fp64", grid_in=None)
     return g.cpu().numpy().reshape(n_fn, N, N), c.cpu().numpy().reshape(n_fn, -1)
 
 
-@pytest.mark.parametrize("B", [1, 3, 6, 16, 21, 64])
+@pytest.mark.parametrize("B", [1, 3, 6, 16, 21, 64, 97, 128])
 def test_s2_complex_fp64(fftso3, torch_cuda, oracle, B):
     F = inputs.s2_coefficients(B, 2, 400 + B)
     g_ref = np.stack([oracle.s2_inverse(B, F[i]) for i in range(2)])
