@@ -555,6 +555,16 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
         // D[N][LT] + spectrum chunk [min(N,128)][ncol] (contract_fwd's KCMAX)
         size_t lds = ((size_t)c->N * a.tile + (size_t)std::min(c->N, 128) * ncol) * sizeof(T);
         hipLaunchKernelGGL((contract_fwd<T, C>), dim3(xcd_grid(a.nitems)), dim3(32 * C), lds, s, a);
+    } else if (sizeof(T) == 4 && c->d_dtab.p) {
+        // FP32 with the d table: MFMA over 128-beta tiles
+        ConArgs<float> af;
+        std::memcpy(&af, &a, sizeof(af));
+        af.items = (const int2 *)c->d_items_inv_mx.p;
+        af.nitems = c->nitems_inv_mx;
+        af.tile = 128;
+        const size_t lds = (size_t)((c->B + 3) & ~3) * TAB_CPITCH * sizeof(float);
+        if (a.s2) hipLaunchKernelGGL((contract_inv_tab<2>), dim3(xcd_grid(af.nitems)), dim3(256), lds, s, af);
+        else hipLaunchKernelGGL((contract_inv_tab<4>), dim3(xcd_grid(af.nitems)), dim3(256), lds, s, af);
     } else {
         a.items = (const int2 *)c->d_items_inv.p;
         a.nitems = c->nitems_inv;
@@ -892,6 +902,8 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32<4, false>));
     ALLOW_LDS((contract_fwd_mx32<2, false>));
     ALLOW_LDS((contract_inv_mx32<4>));
+    ALLOW_LDS((contract_inv_tab<4>));
+    ALLOW_LDS((contract_inv_tab<2>));
     ALLOW_LDS((contract_fwd_mxk<4>));
     ALLOW_LDS((contract_fwd_mxk<2>));
     ALLOW_LDS((contract_inv_mxk<4>));

@@ -607,3 +607,81 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
         }
     }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Inverse contraction of the chunked path at large bandlimits (FP32, Wigner-d table present):
+// workgroup = (group, 128 beta values); the chunk's coefficient block of the group (M rows x up to
+// 64 columns (function, pair, re/im), pair signs applied) is staged in LDS once, the d table rows
+// stream from HBM as the MFMA A operand (32 consecutive k per row: 128-byte reads), wave w owns
+// beta rows [32w, 32w+32) of the tile.  Columns follow the workspace row order, so each output
+// row is one contiguous segment.
+// ---------------------------------------------------------------------------------------------
+#define TAB_CPITCH 65
+
+template <int NP>
+__global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Cb = reinterpret_cast<float *>(smem_raw);   // [M4][TAB_CPITCH]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
+    const int unit = xcd_unit(a.nitems);
+    if (unit >= a.nitems) return;
+    const int2 it = a.items[unit];
+    const int g = it.x, k0 = it.y;
+    int m0, n0;
+    mx_group<NP>(g, m0, n0);
+    const int M = B - m0, M4 = (M + 3) & ~3;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    // stage the coefficient block: rows l - m0 < M4 (zero beyond M), columns c < 64 (zero beyond NCOL)
+    const float2 *Fc = reinterpret_cast<const float2 *>(a.src);
+    for (int e = tid; e < M4 * 64; e += 256) {
+        const int lr = e >> 6, c = e & 63;
+        float v = 0.f;
+        if (lr < M && c < NCOL) {
+            const int p = (c >> 1) % NP, fi = c / (2 * NP);
+            if (mx_pair_valid<NP>(m0, n0, p)) {
+                const long idx = MxPair<NP>(m0, n0, p).packed(m0 + lr);
+                const float2 f = a.pt ? Fc[idx * C + fi] : Fc[ca.coef_off[ca.fn[fi]] + idx];
+                v = (float)mx_pair_sign<NP>(m0, n0, p) * ((c & 1) ? f.y : f.x);
+            }
+        }
+        Cb[lr * TAB_CPITCH + c] = v;
+    }
+    __syncthreads();
+    const float *tab = reinterpret_cast<const float *>(a.dtab) + a.dtab_off[g];
+    const int k = k0 + 32 * wave + cl;           // this lane's A row (beta index)
+    const int kc = k < N ? k : N - 1;            // clamped: rows k >= N are computed, never stored
+    const int nct = NCOL > 32 ? 2 : 1;
+    f32x16m_t acc0, acc1;
+#pragma unroll
+    for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
+    for (int s0 = 0; 2 * s0 < M4; s0 += 8) {
+        float av[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int lr = 2 * (s0 + u) + kh;
+            const float cf = NP == 2 ? 1.f : (float)(2 * (m0 + lr) + 1);
+            av[u] = lr < M ? cf * tab[(long)lr * N + kc] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (2 * (s0 + u) < M4) {
+                const float *cb = Cb + (2 * (s0 + u) + kh) * TAB_CPITCH + cl;
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], cb[0], acc0, 0, 0, 0);
+                if (nct > 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], cb[32], acc1, 0, 0, 0);
+            }
+        }
+    }
+    float *G = reinterpret_cast<float *>(a.dst);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int kk = k0 + 32 * wave + 8 * (j >> 2) + 4 * kh + (j & 3);
+        if (kk < N) {
+            float *row = G + ws_row(ca, g, kk) * CW;
+            if (cl < NCOL) row[cl] = acc0[j];
+            if (nct > 1 && 32 + cl < NCOL) row[32 + cl] = acc1[j];
+        }
+    }
+}

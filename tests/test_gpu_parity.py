@@ -230,3 +230,25 @@ def test_batchwide_large_n_fp32(fftso3, torch_cuda, oracle, domain, B):
         for i in range(n):
             assert rel_l2(g[i], oracle.so3_inverse(B, F[i])) < FP32_TOL
             assert rel_l2(c[i], F[i]) < FP32_TOL
+
+
+@pytest.mark.parametrize("domain,B", [("s2", 150), ("so3", 136)])
+def test_table_path_fp32(fftso3, torch_cuda, oracle, domain, B):
+    """l_max above 127 on the chunked kernels with the plan's Wigner-d table (MFMA inverse),
+    complex and real, against the oracle."""
+    if domain == "s2":
+        F = inputs.s2_coefficients(B, 3, 1600 + B, scale=lambda l: 1.0 / (1.0 + l))
+        g, c = s2_roundtrip(fftso3, torch_cuda, B, F, prec="fp32")
+        for i in range(3):
+            assert rel_l2(g[i], oracle.s2_inverse(B, F[i])) < FP32_TOL
+            assert rel_l2(c[i], F[i]) < FP32_TOL
+        Fr = inputs.s2_coefficients(B, 2, 1700 + B, complex_=False, scale=lambda l: 1.0 / (1.0 + l))
+        g, c = s2_roundtrip(fftso3, torch_cuda, B, Fr, kind="real", prec="fp32")
+        for i in range(2):
+            assert rel_l2(g[i], oracle.s2_real_inverse(B, Fr[i])) < FP32_TOL
+            assert rel_l2(c[i], Fr[i]) < FP32_TOL
+    else:
+        Fr = inputs.so3_coefficients(B, 1, 1600 + B, complex_=False, scale=lambda l: 1.0 / (1.0 + l))
+        g, c = so3_roundtrip(fftso3, torch_cuda, B, Fr, kind="real", prec="fp32")
+        assert rel_l2(g[0], oracle.so3_real_inverse(B, Fr[0])) < 1e-5
+        assert rel_l2(c[0], Fr[0]) < 1e-5
