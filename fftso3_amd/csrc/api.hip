@@ -291,8 +291,9 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     // group's d block resident in LDS (K-chunked kernels)
     {
         const int np = pl->domain == FFTSO3_DOMAIN_S2 ? 2 : 4;
-        const bool fits = B <= MXK_MAXROWS && mxk_lds_bytes(N, B, np, true) <= (size_t)kMaxLds &&
-                          mxk_lds_bytes(N, B, np, false) <= (size_t)kMaxLds;
+        const size_t rcb = (size_t)B * sizeof(RecCoef);
+        const bool fits = B <= MXK_MAXROWS && mxk_lds_bytes(N, B, np, true) + rcb <= (size_t)kMaxLds &&
+                          mxk_lds_bytes(N, B, np, false) + rcb <= (size_t)kMaxLds;
         c->mxk = pl->prec == FFTSO3_FP32 && N > 128 && fits && !pl->force_chunked;
         c->mx = (pl->prec == FFTSO3_FP32 && N <= 128 && !pl->force_chunked) || c->mxk;
     }
@@ -613,7 +614,8 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         a.nsplit = std::max(1, std::min(nfb, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
     }
     const int mgrid = xcd_grid(a.nitems * a.nsplit);
-    const size_t lds = MX_LDS_BYTES;
+    const size_t rcb = (size_t)c->B * sizeof(RecCoef);   // per-group recursion coefficients
+    const size_t lds = MX_LDS_BYTES + rcb;
     a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);
     const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
     const int tgrid = xcd_grid(((ncp + 63) / 64) * ((ca.nfc + 31) / 32));
@@ -632,7 +634,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             a.dst = pl->d_fc.p;
             if (c->mxk)
                 hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(mgrid), dim3(256),
-                                   mxk_lds_bytes(c->N, c->B, NP, true), s, a);
+                                   mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a);
             else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
             else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
             HIP_TRY(hipGetLastError());
@@ -663,7 +665,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         a.dst = pl->d_ws.p;
         if (c->mxk)
             hipLaunchKernelGGL((contract_inv_mxk<NP>), dim3(mgrid), dim3(256),
-                               mxk_lds_bytes(c->N, c->B, NP, false), s, a);
+                               mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a);
         else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
         HIP_TRY(hipGetLastError());
     }

@@ -457,6 +457,67 @@ struct LegendreRec {
     }
 };
 
+// Recursion with its coefficients staged once per group.  The per-l coefficients depend on
+// (m0, n0, l) only, so the workgroup computes them in parallel into LDS (one l per thread) and every
+// beta lane then runs d^l = (a_l cos(beta) - b_l) d^{l-1} - c_l d^{l-2} reading broadcast LDS
+// values instead of chasing table loads through the cache on its sequential path:
+//   SO(3): a_l = l(2l-1) r, b_l = (2l-1)/(l-1) m0 n0 r, c_l = l/(l-1) sqrt((l-1)^2-m0^2) sqrt((l-1)^2-n0^2) r,
+//          r = 1/sqrt((l^2-m0^2)(l^2-n0^2)); l = 1 (m0 = 0): d^1 = cos(beta) d^0.
+//   S^2:   a_l = fact_l, b_l = 0, c_l = fact_l / fact_{l-1} (l - 2 >= m0), as LegendreRec.
+struct RecCoef {
+    double a, b, c;
+};
+
+__device__ __forceinline__ void group_coefs(const RecTables &t, int s2, int B, int m0, int n0, RecCoef *rc, int tid,
+                                            int nthr)
+{
+    const int an0 = iabs(n0);
+    for (int l = m0 + 1 + tid; l < B; l += nthr) {
+        RecCoef r;
+        if (s2) {
+            const double f = t.sq[l * B + m0];
+            r.a = f;
+            r.b = 0.0;
+            r.c = l - 2 >= m0 ? f * t.isq[(l - 1) * B + m0] : 0.0;
+        } else if (l == 1) {
+            r.a = 1.0;
+            r.b = 0.0;
+            r.c = 0.0;
+        } else {
+            const double ir = t.isq[l * B + m0] * t.isq[l * B + an0];
+            r.a = t.lcoef[3 * l] * ir;
+            r.b = t.lcoef[3 * l + 1] * (double)(m0 * n0) * ir;
+            r.c = t.lcoef[3 * l + 2] * t.sq[(l - 1) * B + m0] * t.sq[(l - 1) * B + an0] * ir;
+        }
+        rc[l] = r;
+    }
+}
+
+template <class F>
+__device__ __forceinline__ void group_recursion_lds(const RecTables &t, const RecCoef *rc, int s2, int g, int m0,
+                                                    int n0, int k, int lend, F f)
+{
+    const double cb = t.cb[k];
+    double d1;
+    if (s2) {
+        const double s = exp(t.lnbin[g] + (m0 ? m0 * t.lns[k] : 0.0));
+        d1 = (m0 & 1) ? -s : s;
+    } else {
+        const int a = m0 + n0, b = m0 - n0;
+        const double s = exp(t.lnbin[g] + (a ? a * t.lnc[k] : 0.0) + (b ? b * t.lns[k] : 0.0));
+        d1 = (b & 1) ? -s : s;
+    }
+    double d2 = 0.0;
+    if (m0 < lend) f(m0, d1);
+    for (int l = m0 + 1; l < lend; l++) {
+        const RecCoef r = rc[l];
+        const double v = (r.a * cb - r.b) * d1 - r.c * d2;
+        d2 = d1;
+        d1 = v;
+        f(l, v);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Contraction kernels (vector FMA path; FP32 or FP64 arithmetic)
 // ---------------------------------------------------------------------------------------------

@@ -130,10 +130,13 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     };
     load_block(fb0);
     const int N8 = (N + 7) & ~7;                         // beta extent multiplied (zero columns past N)
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + MX_LDS_BYTES);
+    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 256);
+    __syncthreads();
     for (int k = tid; k < N8; k += 256) {
         if (k < N) {
             const double wk = a.rt.w[k];
-            group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
+            group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * MX_DPITCH + k] = (float)(wk * v);
             });
             for (int r = M; r < 32 * nrt; r++) D[r * MX_DPITCH + k] = 0.f;
@@ -274,9 +277,12 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
         for (int s = 0; s < 32; s++) buf[s] = src[2L * eoff[s] * C];
     };
     load_block(bA, fb0);
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + MX_LDS_BYTES);
+    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 256);
+    __syncthreads();
     for (int k = tid; k < 32 * nkt; k += 256) {
         if (k < N) {
-            group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
+            group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * MX_DPITCH + k] = (float)(NP == 2 ? v : (double)(2 * l + 1) * v);
             });
             for (int r = M; r < M4; r++) D[r * MX_DPITCH + k] = 0.f;
@@ -469,10 +475,13 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
         pf3 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_ + 24, N - 1)) * CW + co_); \
     }
     MXK_LOAD_CHUNK(fb0, 0)
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, B, NP, true));
+    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 256);
+    __syncthreads();
     for (int k = tid; k < N32; k += 256) {
         if (k < N) {
             const double wk = a.rt.w[k];
-            group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
+            group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * DP + k] = (float)(wk * v);
             });
             for (int r = M; r < 32 * nrt; r++) D[r * DP + k] = 0.f;
@@ -559,9 +568,12 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
         const int p = i / MXK_MAXROWS, r = i - p * MXK_MAXROWS;
         etab[i] = MxPair<NP>(m0, n0, p).packed(m0 + min(r, M - 1));
     }
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, B, NP, false));
+    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 256);
+    __syncthreads();
     for (int k = tid; k < 32 * nkt; k += 256) {
         if (k < N) {
-            group_recursion(a.rt, NP == 2, B, g, m0, n0, k, B, [&](int l, double v) {
+            group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * DP + k] = (float)(NP == 2 ? v : (double)(2 * l + 1) * v);
             });
             for (int r = M; r < M4; r++) D[r * DP + k] = 0.f;
