@@ -114,6 +114,7 @@ struct fftso3_plan_s {
     DevBuf d_tmp;       // packed complex coefficients of the batch-wide real path
     size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0, tmp_bytes = 0;
     int klo = 0;                       // row blocking of the batch-wide workspace (ws_row)
+    int gil = 1;                       // group interleave of the batch-wide workspace (ws_rec)
     int split_mul = 1;                 // multiplier of the automatic group split (batch-wide kernels)
     // graph mode: one instantiated graph per (direction, input, output, stream)
     bool graph = false;
@@ -442,8 +443,9 @@ int finalize_plan(fftso3_plan_s *pl)
             if (pl->kind == FFTSO3_REAL) tmpb = std::max(tmpb, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
         }
     }
-    // ws and PT are read up to 512 bytes past a row start (clamped columns of a partial block)
-    ws += 512;
+    // ws and PT are read up to 512 bytes (ws with 4 groups interleaved: 2 KB) past a row start
+    // (clamped columns of a partial block)
+    ws += 4096;
     if (fc) fc += 512;
     if (!pl->d_dummy.p && (st = pl->d_dummy.alloc(256))) return st;
     if (ws != pl->ws_bytes) { pl->d_ws.~DevBuf(); new (&pl->d_ws) DevBuf(); if ((st = pl->d_ws.alloc(ws))) return st; }
@@ -501,6 +503,22 @@ int persistent_grid(K kernel, size_t lds, int nunits, int threads = kFftThreads)
     return std::max(XCD_COUNT, (grid / XCD_COUNT) * XCD_COUNT);
 }
 
+bool g_disable_fft128 = false;
+
+template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128 && !g_disable_fft128; }
+
+// Group interleave of the workspace (ws_rec) for the N = 128 FFT kernels with the batch-wide FP32
+// MFMA contraction and unblocked rows; 0 everywhere else.  Two groups side by side (gil = 1) let
+// each FFT slice fill 64-byte halves of lines by itself while the contraction's row segments
+// still cover half of every line they touch; four (gil = 2) make the FFT's lines whole but
+// quarter the contraction's.
+static int ws_gil(const fftso3_plan_s *pl, const Class *c)
+{
+    const bool ok = pl->prec == FFTSO3_FP32 && c->mx && !c->mxk && c->N == 128 && pl->klo == 0 &&
+                    pl->domain == FFTSO3_DOMAIN_SO3 && !g_disable_fft128;
+    return ok ? pl->gil : 0;
+}
+
 template <typename T>
 ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
 {
@@ -515,6 +533,7 @@ ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
     ca.kmajor = c->mx ? 1 : 0;
     ca.klo = pl->klo;
     ca.ngrp = pl->domain == FFTSO3_DOMAIN_S2 ? c->B : c->B * c->B;
+    ca.gil = ws_gil(pl, c);
     return ca;
 }
 
@@ -698,10 +717,6 @@ int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const v
     default: return launch_contract<T, 8>(pl, c, ca, fwd, src, dst, s);
     }
 }
-
-bool g_disable_fft128 = false;
-
-template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128 && !g_disable_fft128; }
 
 template <typename T>
 int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void *in, void *out, hipStream_t s)
@@ -950,6 +965,7 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
     pl->n_fn = n_fn;
     pl->esz = precision == FFTSO3_FP64 ? 16 : 8;
     pl->klo = ws_klo();
+    if (const char *e = getenv("FFTSO3_WS_GIL")) pl->gil = std::max(0, std::min(3, atoi(e)));
     pl->Bs.resize(n_fn);
     pl->grid_off.resize(n_fn);
     pl->coef_off.resize(n_fn);
@@ -1155,14 +1171,16 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     if (!d_grid || !d_coef) return fail(FFTSO3_EINVAL, "NULL device pointer");
     if (reps < 1 || reps > 100) return fail(FFTSO3_EINVAL, "reps must be in [1, 100] (got %d)", reps);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    struct Cand { bool chunked; int C, klo, split; };
+    struct Cand { bool chunked; int C, klo, split, gil; };
     std::vector<Cand> cands;
     const bool bf16 = plan->prec == FFTSO3_BF16X3 || plan->prec == FFTSO3_BF16X6;
     if (!bf16) {
         if (plan->prec == FFTSO3_FP32)
             for (int klo : {0, 3})
-                for (int sp : {1, 2, 4}) cands.push_back({false, plan->C, klo, sp});
-        for (int C : {1, 2, 4, 8}) cands.push_back({true, C, plan->klo, 1});
+                for (int sp : {1, 2, 4})
+                    for (int gil : {0, 1, 2})
+                        if (klo == 0 || gil == 0) cands.push_back({false, plan->C, klo, sp, gil});
+        for (int C : {1, 2, 4, 8}) cands.push_back({true, C, plan->klo, 1, plan->gil});
     }
     if (cands.empty()) return FFTSO3_OK;
     const bool timing = plan->timing, graph = plan->graph;
@@ -1179,6 +1197,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         plan->C = c.C;
         plan->klo = c.klo;
         plan->split_mul = c.split;
+        plan->gil = c.gil;
         if ((st = finalize_plan(plan))) break;
         if ((st = run_dispatch(plan, false, d_coef, d_grid, s)) || (st = run_dispatch(plan, true, d_grid, d_coef, s))) break;
         (void)hipEventRecord(e0, s);
@@ -1199,6 +1218,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     plan->C = win.C;
     plan->klo = win.klo;
     plan->split_mul = win.split;
+    plan->gil = win.gil;
     const int st2 = finalize_plan(plan);
     plan->timing = timing;
     plan->graph = graph;

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd(FftArgs<float> a)
         float2 p3 = buf[un * P128 + um];     // (n0, m0)
         // the 4 pairs are contiguous: two 16-byte stores
         float4 *S4 = reinterpret_cast<float4 *>(a.dst);
-        const long o = (ws_row(ca, g, k) * ca.C + fi) * 2;
+        const long o = ws_rec(ca, g, k, fi) * 2;
         S4[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
         S4[o + 1] = make_float4(p2.x, p2.y, p3.x, p3.y);
     }
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(1024) so3_fft128_inv(FftArgs<float> a)
     for (int i = 0; i < 4; i++) {
         const int g = tid + i * 1024;
         const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
-        const long o = (ws_row(ca, g, k) * ca.C + fi) * 2;
+        const long o = ws_rec(ca, g, k, fi) * 2;
         q[i][0] = G4[o];
         q[i][1] = G4[o + 1];
     }

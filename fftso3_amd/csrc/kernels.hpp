@@ -38,6 +38,7 @@ struct ChunkArgs {
     int kmajor;             // workspace rows: 0 = [g][k] (chunked kernels), 1 = blocked (batch-wide)
     int klo;                // blocked rows: [k >> klo][g][k & (2^klo - 1)]
     int ngrp;               // (m,n) groups of the bandlimit: B^2 (SO(3)) or B (S^2)
+    int gil;                // log2 of the groups whose records interleave per function (ws_rec)
 };
 
 // Workspace row of (group g, beta index k); a row holds C functions x NP pairs of complex values.
@@ -54,6 +55,16 @@ __device__ __forceinline__ long ws_row(const ChunkArgs &ca, int g, int k)
 __host__ __device__ inline long ws_rows(int kmajor, int klo, int ngrp, int N)
 {
     return kmajor ? (long)(((N + (1 << klo) - 1) >> klo) * ngrp) << klo : (long)ngrp * N;
+}
+
+// Record (NP pairs) of (group g, beta index k, function fi).  With gil > 0 (batch-wide rows,
+// klo = 0) the records of 2^gil consecutive groups sit side by side for each function, so one
+// FFT slice writes (reads) whole 128-byte lines of its own instead of 32-byte pieces that only
+// neighbouring slices complete; the contraction's row segments then stride by 2^gil records.
+__device__ __forceinline__ long ws_rec(const ChunkArgs &ca, int g, int k, int fi)
+{
+    const int gm = (1 << ca.gil) - 1;
+    return ws_row(ca, g & ~gm, k) * ca.C + ((long)fi << ca.gil) + (g & gm);
 }
 
 // workspace element of (group g, beta index k, pair p, function fi)

@@ -119,12 +119,15 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     const int fq = (2 * q) / NP;                         // function (within the block) of the 4 columns
     const int lc0 = mx_lcol<NP>(4 * q), lc1 = mx_lcol<NP>(4 * q + 2);
     const float *S = reinterpret_cast<const float *>(a.src);
+    // record of function fq of the block for this group (ws_rec: 2^gil groups interleaved)
+    const int gm = (1 << ca.gil) - 1;
+    const int qoff = (g & gm) * NP * 2 + (fq << ca.gil) * NP * 2 + (4 * q - fq * NP * 2);
     long roff[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, g, min(r0 + 8 * i, N - 1)) * CW + 4 * q;
+    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, g & ~gm, min(r0 + 8 * i, N - 1)) * CW + qoff;
     float4 pf[16];
     auto load_block = [&](int fb) {
-        const long co = fb * FB + fq < ca.nfc ? (long)fb * 128 : 0;
+        const long co = fb * FB + fq < ca.nfc ? ((long)fb * 128 << ca.gil) : 0;
 #pragma unroll
         for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + roff[i] + co);
     };
@@ -296,9 +299,11 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
     const int fq = (2 * q) / NP;
     const int lc0 = mx_lcol<NP>(4 * q), lc1 = mx_lcol<NP>(4 * q + 2);
     float *G = reinterpret_cast<float *>(a.dst);
+    const int gm = (1 << ca.gil) - 1;
+    const int qoff = (g & gm) * NP * 2 + (fq << ca.gil) * NP * 2 + (4 * q - fq * NP * 2);
     long roff[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) roff[i] = r0 + 8 * i < N ? ws_row(ca, g, r0 + 8 * i) * CW + 4 * q : -1;
+    for (int i = 0; i < 16; i++) roff[i] = r0 + 8 * i < N ? ws_row(ca, g & ~gm, r0 + 8 * i) * CW + qoff : -1;
     auto multiply_store = [&](const float *buf, int fb) {
         f32x16m_t acc[4];
 #pragma unroll
@@ -357,7 +362,7 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
             const float *row = O + (r0 + 8 * i) * MX_LPITCH;
             const float2 u = *reinterpret_cast<const float2 *>(row + lc0);
             const float2 v = *reinterpret_cast<const float2 *>(row + lc1);
-            float *d = ok && roff[i] >= 0 ? G + roff[i] + (long)fb * 128 : a.dummy;
+            float *d = ok && roff[i] >= 0 ? G + roff[i] + ((long)fb * 128 << ca.gil) : a.dummy;
             *reinterpret_cast<float4 *>(d) = make_float4(u.x, u.y, v.x, v.y);
         }
     };
