@@ -637,7 +637,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     const size_t lds = MX_LDS_BYTES + rcb;
     a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);
     const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
-    const int tgrid = xcd_grid(((ncp + 63) / 64) * ((ca.nfc + 31) / 32));
+    const int tgrid = coef_transpose_grid<float2>(ncp, ca.nfc);
     // the real path stages packed complex coefficients in d_tmp, addressed per class-local function
     ChunkArgs ct = ca;
     if (real) {
@@ -779,7 +779,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         }
         if (c->pt) {
             StageTimer t(pl, s, 2);
-            const int tgrid = xcd_grid((int)(((nc + 63) / 64) * ((nfc + 31) / 32)));
+            const int tgrid = coef_transpose_grid<T2>(nc, nfc);
             hipLaunchKernelGGL((coef_transpose<0, T2>), dim3(tgrid), dim3(256), 0, s, ca, (int)nc, pl->d_fc.p, out);
             HIP_TRY(hipGetLastError());
         }
@@ -807,7 +807,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         }
         if (!contracted && c->pt) {
             StageTimer t(pl, s, 3);
-            const int tgrid = xcd_grid((int)(((nc + 63) / 64) * ((nfc + 31) / 32)));
+            const int tgrid = coef_transpose_grid<T2>(nc, nfc);
             hipLaunchKernelGGL((coef_transpose<1, T2>), dim3(tgrid), dim3(256), 0, s, ca, (int)nc, in, pl->d_fc.p);
             HIP_TRY(hipGetLastError());
             csrc = pl->d_fc.p;

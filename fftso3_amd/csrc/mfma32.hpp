@@ -382,46 +382,71 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
 // indices x 32 functions go through LDS so both sides move in contiguous 512- and 256-byte runs.
 // DIR 0: PT -> packed, DIR 1: packed -> PT.
 // ---------------------------------------------------------------------------------------------
+template <typename T2> constexpr int coef_transpose_te() { return sizeof(T2) == 8 ? 128 : 64; }
+
+// launch grid of coef_transpose<., T2> for ncoef packed indices x nfc functions
+template <typename T2> inline int coef_transpose_grid(long ncoef, int nfc)
+{
+    constexpr int TE = coef_transpose_te<T2>();
+    return xcd_grid((int)(((ncoef + TE - 1) / TE) * ((nfc + 31) / 32)));
+}
+
 template <int DIR, typename T2 = float2>
 __global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, const void *src, void *dst)
 {
-    __shared__ T2 tile[32][65];
-    const int nte = (ncoef + 63) / 64, ntf = (ca.nfc + 31) / 32, nunit = nte * ntf;
+    constexpr int TE = coef_transpose_te<T2>(), TF = 32;  // tile: TE packed indices x 32 functions
+    constexpr int RUNS = TE / 64, NJ = 8 * RUNS;          // 64-index runs per row; items per thread
+    __shared__ T2 tile[TF][TE + 1];
+    __shared__ long poff[TF];
+    const int nte = (ncoef + TE - 1) / TE, ntf = (ca.nfc + TF - 1) / TF, nunit = nte * ntf;
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
     const int et = unit / ntf, ft = unit - et * ntf;
-    const int e0 = et * 64, f0 = ft * 32;
+    const int e0 = et * TE, f0 = ft * TF;
     const int t = threadIdx.x;
-    const int fl = t >> 3, eo = (t & 7) * 8;        // packed side: function row, 8 elements
-    const int el = t >> 2, fo = (t & 3) * 8;        // PT side: element row, 8 functions
-    const int fi = f0 + fl;
+    if (t < TF && f0 + t < ca.nfc) poff[t] = ca.coef_off[ca.fn[f0 + t]];
+    __syncthreads();
+    // packed side: wave w moves function rows w + 4j, its 64 lanes runs of 64 consecutive packed
+    // indices; PT side: half-wave h moves element rows h + 8j, its 32 lanes 32 consecutive
+    // functions.  Every instruction moves whole contiguous runs on both sides.
+    const int pe = t & 63, pw = t >> 6;
+    const int qf = t & 31, qh = t >> 5;
+    T2 v[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; j++) v[j] = T2{};
     if (DIR == 1) {
-        if (fi < ca.nfc) {
-            const T2 *P = reinterpret_cast<const T2 *>(src) + ca.coef_off[ca.fn[fi]] + e0 + eo;
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (e0 + eo + j < ncoef) tile[fl][eo + j] = P[j];
+        for (int j = 0; j < NJ; j++) {
+            const int fl = pw + 4 * (j / RUNS), e = e0 + pe + 64 * (j % RUNS);
+            if (f0 + fl < ca.nfc && e < ncoef) v[j] = reinterpret_cast<const T2 *>(src)[poff[fl] + e];
         }
-        __syncthreads();
-        if (e0 + el < ncoef) {
-            T2 *Q = reinterpret_cast<T2 *>(dst) + (long)(e0 + el) * ca.C + f0 + fo;
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (f0 + fo + j < ca.nfc) Q[j] = tile[fo + j][el];
+        for (int j = 0; j < NJ; j++) tile[pw + 4 * (j / RUNS)][pe + 64 * (j % RUNS)] = v[j];
+        __syncthreads();
+        if (f0 + qf < ca.nfc) {
+            T2 *Q = reinterpret_cast<T2 *>(dst) + f0 + qf;
+#pragma unroll
+            for (int j = 0; j < TE / 8; j++) {
+                const int el = qh + 8 * j;
+                if (e0 + el < ncoef) Q[(long)(e0 + el) * ca.C] = tile[qf][el];
+            }
         }
     } else {
-        if (e0 + el < ncoef) {
-            const T2 *Q = reinterpret_cast<const T2 *>(src) + (long)(e0 + el) * ca.C + f0 + fo;
+        if (f0 + qf < ca.nfc) {
+            const T2 *Q = reinterpret_cast<const T2 *>(src) + f0 + qf;
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (f0 + fo + j < ca.nfc) tile[fo + j][el] = Q[j];
+            for (int j = 0; j < TE / 8; j++) {
+                const int el = qh + 8 * j;
+                if (e0 + el < ncoef) v[j] = Q[(long)(e0 + el) * ca.C];
+            }
         }
-        __syncthreads();
-        if (fi < ca.nfc) {
-            T2 *P = reinterpret_cast<T2 *>(dst) + ca.coef_off[ca.fn[fi]] + e0 + eo;
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (e0 + eo + j < ncoef) P[j] = tile[fl][eo + j];
+        for (int j = 0; j < TE / 8; j++) tile[qf][qh + 8 * j] = v[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const int fl = pw + 4 * (j / RUNS), e = e0 + pe + 64 * (j % RUNS);
+            if (f0 + fl < ca.nfc && e < ncoef) reinterpret_cast<T2 *>(dst)[poff[fl] + e] = tile[fl][pe + 64 * (j % RUNS)];
         }
     }
 }
