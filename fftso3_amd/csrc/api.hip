@@ -583,14 +583,16 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
         af.nitems = c->nitems_inv_mx;
         af.tile = 128;
         const size_t lds = (size_t)((c->B + 3) & ~3) * TAB_CPITCH * sizeof(float);
-        if (a.s2) hipLaunchKernelGGL((contract_inv_tab<2>), dim3(xcd_grid(af.nitems)), dim3(256), lds, s, af);
-        else hipLaunchKernelGGL((contract_inv_tab<4>), dim3(xcd_grid(af.nitems)), dim3(256), lds, s, af);
+        const int tgrid = deal_grid(af.nitems, (c->N + 127) / 128);
+        if (a.s2) hipLaunchKernelGGL((contract_inv_tab<2>), dim3(tgrid), dim3(256), lds, s, af);
+        else hipLaunchKernelGGL((contract_inv_tab<4>), dim3(tgrid), dim3(256), lds, s, af);
     } else {
         a.items = (const int2 *)c->d_items_inv.p;
         a.nitems = c->nitems_inv;
         a.tile = c->kt_inv;
         size_t lds = ((size_t)c->B * a.tile + (size_t)c->B * ncol) * sizeof(T);
-        hipLaunchKernelGGL((contract_inv<T, C>), dim3(xcd_grid(a.nitems)), dim3(32 * C), lds, s, a);
+        hipLaunchKernelGGL((contract_inv<T, C>), dim3(deal_grid(a.nitems, (c->N + a.tile - 1) / a.tile)), dim3(32 * C),
+                           lds, s, a);
     }
     HIP_TRY(hipGetLastError());
     return FFTSO3_OK;
@@ -632,7 +634,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         const int nfb = (ca.nfc * NP * 2 + 127) / 128;
         a.nsplit = std::max(1, std::min(nfb, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
     }
-    const int mgrid = xcd_grid(a.nitems * a.nsplit);
+    const int mgrid = deal_grid(a.nitems * a.nsplit, a.nsplit << ca.gil);
     const size_t rcb = (size_t)c->B * sizeof(RecCoef);   // per-group recursion coefficients
     const size_t lds = MX_LDS_BYTES + rcb;
     a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);

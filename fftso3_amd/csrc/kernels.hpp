@@ -25,6 +25,21 @@ __device__ __forceinline__ int xcd_unit(int nunits)
 }
 static inline int xcd_grid(int nunits) { return ((nunits + XCD_COUNT - 1) / XCD_COUNT) * XCD_COUNT; }
 
+// Map blockIdx.x for units of unequal cost (groups come largest first): runs of `run`
+// consecutive units are dealt round-robin over the XCDs, so every XCD gets an even share of the
+// long units while each run (units that share cache lines) stays under one L2.  The grid is
+// deal_grid(n, run); units >= n exit.
+__device__ __forceinline__ int xcd_deal(int run)
+{
+    const int x = blockIdx.x % XCD_COUNT, i = blockIdx.x / XCD_COUNT;
+    return ((i / run) * XCD_COUNT + x) * run + (i % run);
+}
+static inline int deal_grid(int nunits, int run)
+{
+    const int q = XCD_COUNT * run;
+    return ((nunits + q - 1) / q) * q;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Per-chunk addressing shared by all kernels
 // ---------------------------------------------------------------------------------------------
@@ -720,7 +735,7 @@ __global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, KT = a.tile;
-    const int unit = xcd_unit(a.nitems);
+    const int unit = xcd_deal((a.ca.N + a.tile - 1) / a.tile);  // runs: the k tiles of one group
     if (unit >= a.nitems) return;
     const int2 it = a.items[unit];
     const int g = it.x, k0 = it.y;

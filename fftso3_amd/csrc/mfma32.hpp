@@ -44,7 +44,9 @@ struct MxArgs {
 __device__ __forceinline__ bool mx_unit(const MxArgs &a, int nfb, int &g, int &fb0, int &fb1)
 {
     const int n = a.nitems * a.nsplit;
-    const int unit = xcd_unit(n);
+    // groups come largest first: deal them round-robin over the XCDs (xcd_deal), in runs of the
+    // nsplit units of a group times the 2^gil groups whose workspace records share lines
+    const int unit = xcd_deal(a.nsplit << a.ca.gil);
     if (unit >= n) return false;
     const int it = unit / a.nsplit, part = unit - it * a.nsplit;
     g = a.items[it];
@@ -667,7 +669,7 @@ __global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
     float *Cb = reinterpret_cast<float *>(smem_raw);   // [M4][TAB_CPITCH]
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
-    const int unit = xcd_unit(a.nitems);
+    const int unit = xcd_deal((a.ca.N + 127) / 128);
     if (unit >= a.nitems) return;
     const int2 it = a.items[unit];
     const int g = it.x, k0 = it.y;
