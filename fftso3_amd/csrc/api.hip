@@ -554,6 +554,16 @@ FftArgs<T> fft_args(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, const void
 }
 
 
+// table-fed MFMA forward contraction above l_max = 127 (FFTSO3_FWD_TAB=0 selects the vector kernel)
+static bool fwd_tab()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FWD_TAB");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 template <typename T, int C>
 int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst,
                     hipStream_t s)
@@ -568,7 +578,16 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
     a.src = src;
     a.dst = dst;
     const int np = a.s2 ? 2 : 4, ncol = 2 * np * C;
-    if (fwd) {
+    if (fwd && sizeof(T) == 4 && c->d_dtab.p && ncol <= TABF_COLS && (size_t)c->N * TABF_COLS * 4 <= (size_t)kMaxLds &&
+        fwd_tab()) {
+        // FP32 with the d table: MFMA over 32-degree tiles, one workgroup per group
+        ConArgs<float> af;
+        std::memcpy(&af, &a, sizeof(af));
+        const int ngrp = a.s2 ? c->B : c->B * c->B;
+        const size_t lds = (size_t)c->N * TABF_COLS * sizeof(float);
+        if (a.s2) hipLaunchKernelGGL((contract_fwd_tab<2>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+        else hipLaunchKernelGGL((contract_fwd_tab<4>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+    } else if (fwd) {
         a.items = (const int2 *)c->d_items_fwd.p;
         a.nitems = c->nitems_fwd;
         a.tile = c->lt_fwd;
@@ -917,6 +936,8 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv<double, 4>));
     ALLOW_LDS((contract_inv<double, 8>));
     ALLOW_LDS((contract_fwd_mx32<4, true>));
+    ALLOW_LDS((contract_fwd_tab<4>));
+    ALLOW_LDS((contract_fwd_tab<2>));
     ALLOW_LDS((contract_fwd_mx32<2, true>));
     ALLOW_LDS((contract_fwd_mx32<4, false>));
     ALLOW_LDS((contract_fwd_mx32<2, false>));

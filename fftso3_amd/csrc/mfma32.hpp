@@ -660,6 +660,89 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
 // beta rows [32w, 32w+32) of the tile.  Columns follow the workspace row order, so each output
 // row is one contiguous segment.
 // ---------------------------------------------------------------------------------------------
+// Forward on the d table (chunked path above l_max = 127, FP32): one workgroup per group.  The
+// group's spectrum block (N beta rows x the chunk's <= 64 columns, scaled by w_k) is staged in LDS
+// once; each wave then takes 32-degree row tiles and multiplies table rows (A operand, FP32 from
+// HBM/L2) by the block on FP32 MFMA.  Step s of a tile covers beta rows s (lanes 0-31) and
+// s + N/2 (lanes 32-63); the second half of the rows is stored 32 columns rotated so the two
+// halves' operand reads fall in opposite LDS banks.
+#define TABF_COLS 64
+
+template <int NP>
+__global__ void __launch_bounds__(256) contract_fwd_tab(ConArgs<float> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Sl = reinterpret_cast<float *>(smem_raw);    // [N][64]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2, H = N / 2;
+    const int g = xcd_deal(1);
+    if (g >= ca.ngrp) return;
+    int m0, n0;
+    mx_group<NP>(g, m0, n0);
+    const int M = B - m0, nt = (M + 31) / 32;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    // stage w_k S_k: CW / 4 float4 per beta row; row k >= H rotated by 32 columns
+    {
+        const float4 *S4 = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(a.src) + ws_row(ca, g, 0) * CW);
+        const int q4 = CW / 4;
+        for (int e = tid; e < N * q4; e += 256) {
+            const int k = e / q4, c = 4 * (e - k * q4);
+            float4 v = S4[e];
+            const double w = a.rt.w[k];
+            v = make_float4((float)(w * v.x), (float)(w * v.y), (float)(w * v.z), (float)(w * v.w));
+            *reinterpret_cast<float4 *>(Sl + k * TABF_COLS + ((c + (k >= H ? 32 : 0)) & (TABF_COLS - 1))) = v;
+        }
+    }
+    __syncthreads();
+    const float *tab = reinterpret_cast<const float *>(a.dtab) + a.dtab_off[g];
+    const int nct = NCOL > 32 ? 2 : 1;
+    // this lane's output column: function fi, pair p, re/im
+    const int c0 = cl, c1 = 32 + cl;
+    const float *Sb0 = Sl + kh * H * TABF_COLS + ((c0 + 32 * kh) & (TABF_COLS - 1));
+    const float *Sb1 = Sl + kh * H * TABF_COLS + ((c1 + 32 * kh) & (TABF_COLS - 1));
+    float *dst = reinterpret_cast<float *>(a.dst);
+    const long es = a.pt ? 2L * C : 2L;
+    for (int t = wave; t < nt; t += 4) {
+        const int lr = 32 * t + cl;                       // this lane's A row (degree m0 + lr)
+        const float *trow = tab + (long)min(lr, M - 1) * N + kh * H;   // rows >= M: finite, never stored
+        f32x16m_t acc0, acc1;
+#pragma unroll
+        for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
+        int s0 = 0;
+        for (; s0 + 8 <= H; s0 += 8) {
+            float av[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) av[u] = trow[s0 + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], Sb0[(s0 + u) * TABF_COLS], acc0, 0, 0, 0);
+                if (nct > 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], Sb1[(s0 + u) * TABF_COLS], acc1, 0, 0, 0);
+            }
+        }
+        for (; s0 < H; s0++) {
+            const float av = trow[s0];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Sb0[s0 * TABF_COLS], acc0, 0, 0, 0);
+            if (nct > 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Sb1[s0 * TABF_COLS], acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int c = h ? c1 : c0;
+            if (c >= NCOL || (h && nct < 2)) continue;
+            const int ri = c & 1, p = (c >> 1) % NP, fi = c / (2 * NP);
+            if (!mx_pair_valid<NP>(m0, n0, p)) continue;
+            const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
+            const MxPair<NP> pr(m0, n0, p);
+            float *out = dst + (a.pt ? 2L * fi : 2L * ca.coef_off[ca.fn[fi]]) + ri;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int r = 32 * t + 8 * (j >> 2) + 4 * kh + (j & 3);
+                if (r < M) out[es * pr.packed(m0 + r)] = sg * (h ? acc1[j] : acc0[j]);
+            }
+        }
+    }
+}
+
 #define TAB_CPITCH 65
 
 template <int NP>
