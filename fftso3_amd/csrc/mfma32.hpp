@@ -685,13 +685,28 @@ __global__ void __launch_bounds__(256) contract_fwd_tab(ConArgs<float> a)
     // stage w_k S_k: CW / 4 float4 per beta row; row k >= H rotated by 32 columns
     {
         const float4 *S4 = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(a.src) + ws_row(ca, g, 0) * CW);
-        const int q4 = CW / 4;
-        for (int e = tid; e < N * q4; e += 256) {
-            const int k = e / q4, c = 4 * (e - k * q4);
-            float4 v = S4[e];
-            const double w = a.rt.w[k];
-            v = make_float4((float)(w * v.x), (float)(w * v.y), (float)(w * v.z), (float)(w * v.w));
-            *reinterpret_cast<float4 *>(Sl + k * TABF_COLS + ((c + (k >= H ? 32 : 0)) & (TABF_COLS - 1))) = v;
+        const int q4 = CW / 4, ne = N * q4;
+        for (int e0 = 0; e0 < ne; e0 += 256 * 8) {
+            float4 v[8];
+            double w[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {                 // 8 loads in flight per thread
+                const int e = e0 + tid + 256 * i;
+                if (e < ne) {
+                    v[i] = S4[e];
+                    w[i] = a.rt.w[e / q4];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int e = e0 + tid + 256 * i;
+                if (e < ne) {
+                    const int k = e / q4, c = 4 * (e - k * q4);
+                    const float4 u = make_float4((float)(w[i] * v[i].x), (float)(w[i] * v[i].y), (float)(w[i] * v[i].z),
+                                                 (float)(w[i] * v[i].w));
+                    *reinterpret_cast<float4 *>(Sl + k * TABF_COLS + ((c + (k >= H ? 32 : 0)) & (TABF_COLS - 1))) = u;
+                }
+            }
         }
     }
     __syncthreads();
@@ -710,6 +725,28 @@ __global__ void __launch_bounds__(256) contract_fwd_tab(ConArgs<float> a)
 #pragma unroll
         for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
         int s0 = 0;
+        if ((H & 31) == 0 && (N & 3) == 0) {
+            // table rows as float4, 32 steps per round with the next round's 8 loads in flight
+            const float4 *t4 = reinterpret_cast<const float4 *>(trow);
+            float4 cur[8], nxt[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) cur[i] = t4[i];
+            for (; s0 < H; s0 += 32) {
+                if (s0 + 32 < H) {
+#pragma unroll
+                    for (int i = 0; i < 8; i++) nxt[i] = t4[(s0 + 32) / 4 + i];
+                }
+#pragma unroll
+                for (int u = 0; u < 32; u++) {
+                    const float4 &q = cur[u >> 2];
+                    const float av = (u & 3) == 0 ? q.x : (u & 3) == 1 ? q.y : (u & 3) == 2 ? q.z : q.w;
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Sb0[(s0 + u) * TABF_COLS], acc0, 0, 0, 0);
+                    if (nct > 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, Sb1[(s0 + u) * TABF_COLS], acc1, 0, 0, 0);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; i++) cur[i] = nxt[i];
+            }
+        }
         for (; s0 + 8 <= H; s0 += 8) {
             float av[8];
 #pragma unroll
