@@ -82,6 +82,32 @@ __device__ __forceinline__ void rows_stageB(float2 *buf, int tid)
     for (int q = 0; q < 16; q++) buf[row * P128 + s + 8 * q] = v[q];
 }
 
+// Column pass whose stage B hands X[s + 8q][col] to out(row, col, value) instead of writing LDS
+// (lanes of a wave hold 64 consecutive columns of one row: the inverse writes them straight to
+// the grid).
+template <typename F>
+__device__ __forceinline__ void cols_pass_out(float2 *buf, const float2 *twl, int tid, F out)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int t = tid + i * 1024, col = t & 127, j = t >> 7;
+        float2 v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = buf[(j + 16 * r) * P128 + col];
+        stageA(v, j, twl);
+#pragma unroll
+        for (int s = 0; s < 8; s++) buf[(j + 16 * s) * P128 + col] = v[s];
+    }
+    __syncthreads();
+    const int col = tid & 127, s = tid >> 7;
+    float2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = buf[(16 * s + q) * P128 + col];
+    dft16(v);
+#pragma unroll
+    for (int q = 0; q < 16; q++) out(s + 8 * q, col, v[q]);
+}
+
 __device__ __forceinline__ void cols_pass(float2 *buf, const float2 *twl, int tid)
 {
 #pragma unroll
@@ -215,23 +241,13 @@ __global__ void __launch_bounds__(1024) so3_fft128_inv(FftArgs<float> a)
     __syncthreads();
     rows_stageB(buf, tid);
     __syncthreads();
-    cols_pass(buf, twl, tid);
+    // last stage straight to the grid: each store instruction covers 64 consecutive gamma columns
     const long base = ca.grid_off[ca.fn[fi]] + (long)k * F128 * F128;
     if (REAL) {
         float *out = reinterpret_cast<float *>(a.dst) + base;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int e4 = tid + i * 1024, row = e4 >> 5, c4 = (e4 & 31) * 4;  // 4 reals per lane
-            const float2 *r = buf + row * P128 + c4;
-            reinterpret_cast<float4 *>(out)[e4] = make_float4(r[0].x, r[1].x, r[2].x, r[3].x);
-        }
+        cols_pass_out(buf, twl, tid, [&](int row, int col, float2 v) { out[row * F128 + col] = v.x; });
     } else {
-        float4 *out = reinterpret_cast<float4 *>(reinterpret_cast<float2 *>(a.dst) + base);
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int e2 = tid + i * 1024, row = e2 >> 6, c2 = (e2 & 63) * 2;   // 2 complex per lane
-            const float2 *r = buf + row * P128 + c2;
-            out[e2] = make_float4(r[0].x, r[0].y, r[1].x, r[1].y);
-        }
+        float2 *out = reinterpret_cast<float2 *>(a.dst) + base;
+        cols_pass_out(buf, twl, tid, [&](int row, int col, float2 v) { out[row * F128 + col] = v; });
     }
 }
