@@ -638,6 +638,16 @@ int launch_contract_bf16(Class *c, const ConArgs<float> &a0, bool fwd, hipStream
     return FFTSO3_OK;
 }
 
+// grouped forward contraction on the interleaved workspace (FFTSO3_FWD_GROUPED=0: one group per workgroup)
+static bool fwd_grouped()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FWD_GROUPED");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 template <int NP>
 int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *in, void *out, hipStream_t s)
 {
@@ -648,6 +658,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     a.rt = rec_tables(c);
     a.items = (const int *)c->d_items_mx.p;
     a.nitems = c->nitems_mx;
+    a.gpw = 1;
     {
         // split each group's column blocks over several workgroups when there are few groups
         const int nfb = (ca.nfc * NP * 2 + 127) / 128;
@@ -675,7 +686,18 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             if (c->mxk)
                 hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(mgrid), dim3(256),
                                    mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a);
-            else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
+            else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2 && fwd_grouped()) {
+                // the 2^gil groups that share workspace lines in one workgroup
+                MxArgs ag = a;
+                ag.gpw = 1 << ca.gil;
+                ag.nitems = c->nitems_mx / ag.gpw;
+                const int fbw = 16 / ag.gpw, nfbg = (ca.nfc + fbw - 1) / fbw;
+                ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
+                const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
+                const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
+                if (ag.gpw == 2) hipLaunchKernelGGL((contract_fwd_mx32g<2>), dim3(ggrid), dim3(256), glds, s, ag);
+                else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
+            } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
             else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
             HIP_TRY(hipGetLastError());
         }
@@ -936,6 +958,8 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv<double, 4>));
     ALLOW_LDS((contract_inv<double, 8>));
     ALLOW_LDS((contract_fwd_mx32<4, true>));
+    ALLOW_LDS((contract_fwd_mx32g<2>));
+    ALLOW_LDS((contract_fwd_mx32g<4>));
     ALLOW_LDS((contract_fwd_tab<4>));
     ALLOW_LDS((contract_fwd_tab<2>));
     ALLOW_LDS((contract_fwd_mx32<2, true>));

@@ -33,6 +33,7 @@ struct MxArgs {
     const int *items;        // groups (largest first)
     int nitems;
     int nsplit;              // workgroups per group, each taking a contiguous range of column blocks
+    int gpw;                 // groups per workgroup (contract_fwd_mx32g: items are sets of gpw groups)
     const void *src;         // forward: spectrum workspace; inverse: PT
     void *dst;               // forward: PT; inverse: spectrum workspace
     float *dummy;            // 16-byte slot that absorbs the stores of lanes with nothing to store
@@ -46,10 +47,10 @@ __device__ __forceinline__ bool mx_unit(const MxArgs &a, int nfb, int &g, int &f
     const int n = a.nitems * a.nsplit;
     // groups come largest first: deal them round-robin over the XCDs (xcd_deal), in runs of the
     // nsplit units of a group times the 2^gil groups whose workspace records share lines
-    const int unit = xcd_deal(a.nsplit << a.ca.gil);
+    const int unit = xcd_deal(a.gpw > 1 ? a.nsplit : a.nsplit << a.ca.gil);
     if (unit >= n) return false;
     const int it = unit / a.nsplit, part = unit - it * a.nsplit;
-    g = a.items[it];
+    g = a.gpw > 1 ? it * a.gpw : a.items[it];
     fb0 = (int)((long)part * nfb / a.nsplit);
     fb1 = (int)((long)(part + 1) * nfb / a.nsplit);
     return fb0 < fb1;
@@ -232,6 +233,137 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
 #pragma unroll
                 for (int u = 0; u < 4; u++) acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(d0[u], b[u], acc0, 0, 0, 0);
             }
+        }
+        const int fn = fb * FB + fo;
+        const bool ok = pvalid && fn < ca.nfc;
+        float *o = PT + 2 * fn + (cl & 1);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int r = 8 * (j >> 2) + 4 * kh + (j & 3);
+            const long e0 = pr.packed(m0 + min(r, M - 1)), e1 = pr.packed(m0 + min(r + 32, M - 1));
+            float *d0 = ok && r < M ? o + 2 * e0 * C : a.dummy;
+            float *d1 = ok && r + 32 < M ? o + 2 * e1 * C : a.dummy;
+            *d0 = sg * acc0[j];
+            *d1 = sg * acc1[j];
+        }
+    }
+}
+
+// Forward, N = 128, SO(3), workspace with 2^gil groups interleaved (GPW = 2^gil): one workgroup
+// takes the GPW groups whose records share each workspace line, so a block (16 / GPW functions of
+// all GPW groups) is again 512 contiguous bytes of every beta row.  Wave w works on group
+// w / (4 / GPW) and on GPW of its pairs, 32 columns as before; each wave keeps its own group's
+// D in registers, staged two groups at a time through the LDS that later holds the blocks.
+template <int GPW>
+__global__ void __launch_bounds__(256) contract_fwd_mx32g(MxArgs a)
+{
+    constexpr int NP = 4, FB = 16 / GPW, WPG = 4 / GPW;  // functions per block, waves per group
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *L = reinterpret_cast<float *>(smem_raw);     // [128][MX_LPITCH], spectrum block
+    float *Dst = L;                                      // [2][64][MX_DPITCH] staging, before L is used
+    RecCoef *rcs = reinterpret_cast<RecCoef *>(smem_raw + 128 * MX_LPITCH * 4);   // [2][B]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
+    const int nfb = (ca.nfc + FB - 1) / FB;
+    int gb, fb0, fb1;
+    if (!mx_unit(a, nfb, gb, fb0, fb1)) return;         // gb: first group of the interleaved set
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    const int gg = wave / WPG, pg = wave % WPG;          // this wave's group and pair set
+    const int g = gb + gg;
+    int m0, n0;
+    so3_group(g, m0, n0);
+    const int M = B - m0, nrt = (M + 31) / 32;
+    // load role: rows r0 + 8i (i < 16) of the block, columns 4q..4q+3 = record (fl, gq), pairs
+    // 2(q & 1), 2(q & 1) + 1
+    const int q = tid & 31, r0 = tid >> 5;
+    const int fl_q = (q >> 1) / GPW, gq = (q >> 1) % GPW;
+    auto lcol = [](int fl, int gq_, int p) {            // LDS column of (function, group, pair), re part
+        return (gq_ * WPG + p / GPW) * 32 + (p % GPW) * FB * 2 + fl * 2;
+    };
+    const int lc0 = lcol(fl_q, gq, 2 * (q & 1)), lc1 = lcol(fl_q, gq, 2 * (q & 1) + 1);
+    const float *S = reinterpret_cast<const float *>(a.src);
+    long roff[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, gb, min(r0 + 8 * i, N - 1)) * CW + 4 * q;
+    float4 pf[16];
+    auto load_block = [&](int fb) {
+        const long co = fb * FB + fl_q < ca.nfc ? (long)fb * 128 : 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + roff[i] + co);
+    };
+    load_block(fb0);
+    // D of the GPW groups, two at a time: threads 0-127 recurse for the first, 128-255 the second
+    float dA[64], dB[64];
+    for (int ph = 0; ph < (GPW + 1) / 2; ph++) {
+        const int h = tid >> 7, gs = 2 * ph + h;        // staged group (index within the set)
+        const bool active = gs < GPW;
+        int sm0 = 0, sn0 = 0;
+        if (active) so3_group(gb + gs, sm0, sn0);
+        RecCoef *rc = rcs + h * B;
+        float *Dh = Dst + h * 64 * MX_DPITCH;
+        if (active) group_coefs(a.rt, 0, B, sm0, sn0, rc, tid & 127, 128);
+        __syncthreads();
+        if (active) {
+            const int k = tid & 127, sM = B - sm0, snrt = (sM + 31) / 32;
+            const double wk = a.rt.w[k];
+            group_recursion_lds(a.rt, rc, 0, gb + gs, sm0, sn0, k, B, [&](int l, double v) {
+                Dh[(l - sm0) * MX_DPITCH + k] = (float)(wk * v);
+            });
+            for (int r = sM; r < 32 * snrt; r++) Dh[r * MX_DPITCH + k] = 0.f;
+        }
+        __syncthreads();
+        if (gg / 2 == ph) {
+            const float *D0 = Dst + (gg & 1) * 64 * MX_DPITCH + cl * MX_DPITCH + kh;
+#pragma unroll
+            for (int s = 0; s < 64; s++) {
+                dA[s] = D0[2 * s];
+                dB[s] = nrt > 1 ? D0[32 * MX_DPITCH + 2 * s] : 0.f;
+            }
+        }
+        __syncthreads();                                 // staging read before it is overwritten
+    }
+    // output role of this lane: pair p, function fo of the block, re/im cl & 1
+    const int p = pg * GPW + cl / (FB * 2), fo = (cl % (FB * 2)) >> 1;
+    const float sg = (float)so3_pair_sign(m0, n0, p);
+    const bool pvalid = so3_pair_valid(m0, n0, p);
+    const MxPair<NP> pr(m0, n0, p);
+    float *PT = reinterpret_cast<float *>(a.dst);
+    const float *Lw = L + kh * MX_LPITCH + wave * 32 + cl;
+    for (int fb = fb0; fb < fb1; fb++) {
+        __syncthreads();                                 // previous block consumed
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            float *row = L + (r0 + 8 * i) * MX_LPITCH;
+            *reinterpret_cast<float2 *>(row + lc0) = make_float2(pf[i].x, pf[i].y);
+            *reinterpret_cast<float2 *>(row + lc1) = make_float2(pf[i].z, pf[i].w);
+        }
+        __syncthreads();
+        if (fb + 1 < fb1) load_block(fb + 1);
+        f32x16m_t acc0, acc1;
+#pragma unroll
+        for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
+        if (nrt > 1) {
+#pragma unroll
+            for (int s0 = 0; s0 < 64; s0 += 8) {
+                float b[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) b[u] = Lw[2 * (s0 + u) * MX_LPITCH];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(dA[s0 + u], b[u], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(dB[s0 + u], b[u], acc1, 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 64; s += 2) {
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(dA[s], Lw[2 * s * MX_LPITCH], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(dA[s + 1], Lw[(2 * s + 2) * MX_LPITCH], acc1, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; j++) { acc0[j] += acc1[j]; acc1[j] = 0.f; }
         }
         const int fn = fb * FB + fo;
         const bool ok = pvalid && fn < ca.nfc;
