@@ -80,6 +80,12 @@ public:
     // values of one function at n points (angles: 3 or 2 doubles per point), host arrays
     void evaluate(int domain, int l_max, int kind, int precision, const void *coef, const double *angles, long n,
                   void *values);
+    // Forward transform of one function sampled on the host: fill(i0, i1, out) writes grid values
+    // [i0, i1) (grid order; complex as re, im) as doubles and is called concurrently from OpenMP
+    // threads.  Sampling runs chunk by chunk into pinned buffers while the previous chunk is copied
+    // to the device on a stream of the plan's own.
+    using Fill = std::function<void(long i0, long i1, double *out)>;
+    void forward_fill(int domain, int l_max, int kind, int precision, const Fill &fill, void *coef);
 
 private:
     void prepare(int domain, int l_max, int n_fn, int kind, int precision);
@@ -90,6 +96,9 @@ private:
     std::vector<char> stage_;  // host staging for the single-precision plans
     void *d_ang_ = nullptr, *d_val_ = nullptr;
     long pts_cap_ = 0;
+    void *pin_[2] = {nullptr, nullptr};  // pinned sampling buffers of forward_fill
+    size_t pin_bytes_ = 0;
+    void *stream_ = nullptr, *ev_[2] = {nullptr, nullptr};
 };
 }  // namespace detail
 

@@ -75,12 +75,16 @@ FFTS2_matrix_complex FFTS2_complex::spherical_harmonics(double theta, double phi
 
 FFTS2_matrix_complex FFTS2_complex::forward_transform(std::function<complex<double>(double, double)> func)
 {
-    const int N = 2 * B;
-    std::vector<complex<double>> grid((size_t)N * N);
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int k = 0; k < N; k++)
-        for (int j = 0; j < N; j++) grid[(size_t)k * N + j] = func(theta_k(k), phi_j(j));
-    return forward_transform(grid);
+    const long N = 2 * B;
+    FFTS2_matrix_complex F(l_max);
+    dev_->forward_fill(FFTSO3_DOMAIN_S2, l_max, FFTSO3_COMPLEX, precision_, [&](long i0, long i1, double *out) {
+        for (long i = i0; i < i1; i++) {
+            const complex<double> v = func(theta_k((int)(i / N)), phi_j((int)(i % N)));
+            out[2 * (i - i0)] = v.real();
+            out[2 * (i - i0) + 1] = v.imag();
+        }
+    }, F.v.data());
+    return F;
 }
 
 FFTS2_matrix_complex FFTS2_complex::forward_transform(const std::vector<complex<double>> &grid)
@@ -204,12 +208,12 @@ FFTS2_matrix_real FFTS2_real::spherical_harmonics(double theta, double phi, int 
 
 FFTS2_matrix_real FFTS2_real::forward_transform(std::function<double(double, double)> func)
 {
-    const int N = 2 * B;
-    std::vector<double> grid((size_t)N * N);
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int k = 0; k < N; k++)
-        for (int j = 0; j < N; j++) grid[(size_t)k * N + j] = func(theta_k(k), phi_j(j));
-    return forward_transform(grid);
+    const long N = 2 * B;
+    FFTS2_matrix_real F(l_max);
+    dev_->forward_fill(FFTSO3_DOMAIN_S2, l_max, FFTSO3_REAL, precision_, [&](long i0, long i1, double *out) {
+        for (long i = i0; i < i1; i++) out[i - i0] = func(theta_k((int)(i / N)), phi_j((int)(i % N)));
+    }, F.v.data());
+    return F;
 }
 
 FFTS2_matrix_real FFTS2_real::forward_transform(const std::vector<double> &grid)

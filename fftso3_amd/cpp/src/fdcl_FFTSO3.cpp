@@ -135,16 +135,21 @@ std::vector<double> FFTSO3_complex::character(double theta)
     return chi;
 }
 
+// f is sampled on the host by OpenMP threads (as src/fdcl_FFTSO3.cpp:675-685 does), chunk by chunk
+// into pinned buffers whose copies to the device overlap the sampling of the next chunk
 FFTSO3_matrix_complex FFTSO3_complex::forward_transform(std::function<complex<double>(double, double, double)> func)
 {
-    const int N = 2 * B;
-    std::vector<complex<double>> grid((size_t)N * N * N);
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int k = 0; k < N; k++)
-        for (int j1 = 0; j1 < N; j1++)
-            for (int j2 = 0; j2 < N; j2++)
-                grid[((size_t)k * N + j1) * N + j2] = func(alpha_j(j1), beta_k(k), gamma_j(j2));
-    return forward_transform(grid);
+    const long N = 2 * B;
+    FFTSO3_matrix_complex F(l_max);
+    dev_->forward_fill(FFTSO3_DOMAIN_SO3, l_max, FFTSO3_COMPLEX, precision_, [&](long i0, long i1, double *out) {
+        for (long i = i0; i < i1; i++) {
+            const long k = i / (N * N), j1 = (i / N) % N, j2 = i % N;
+            const complex<double> v = func(alpha_j((int)j1), beta_k((int)k), gamma_j((int)j2));
+            out[2 * (i - i0)] = v.real();
+            out[2 * (i - i0) + 1] = v.imag();
+        }
+    }, F.v.data());
+    return F;
 }
 
 FFTSO3_matrix_complex FFTSO3_complex::forward_transform(std::function<complex<double>(Matrix3d)> func)
@@ -358,14 +363,15 @@ FFTSO3_matrix_real FFTSO3_real::real_harmonics(const Matrix3d &R)
 
 FFTSO3_matrix_real FFTSO3_real::forward_transform(std::function<double(double, double, double)> func)
 {
-    const int N = 2 * B;
-    std::vector<double> grid((size_t)N * N * N);
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int k = 0; k < N; k++)
-        for (int j1 = 0; j1 < N; j1++)
-            for (int j2 = 0; j2 < N; j2++)
-                grid[((size_t)k * N + j1) * N + j2] = func(alpha_j(j1), beta_k(k), gamma_j(j2));
-    return forward_transform(grid);
+    const long N = 2 * B;
+    FFTSO3_matrix_real F(l_max);
+    dev_->forward_fill(FFTSO3_DOMAIN_SO3, l_max, FFTSO3_REAL, precision_, [&](long i0, long i1, double *out) {
+        for (long i = i0; i < i1; i++) {
+            const long k = i / (N * N), j1 = (i / N) % N, j2 = i % N;
+            out[i - i0] = func(alpha_j((int)j1), beta_k((int)k), gamma_j((int)j2));
+        }
+    }, F.v.data());
+    return F;
 }
 
 FFTSO3_matrix_real FFTSO3_real::forward_transform(std::function<double(Matrix3d)> func)

@@ -2,6 +2,7 @@
 // and the Euler-angle helpers shared by the SO(3) classes.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -62,11 +63,17 @@ static void check_hip(hipError_t e, const char *what)
 
 DevicePlan::~DevicePlan()
 {
+    if (stream_) (void)hipStreamSynchronize(static_cast<hipStream_t>(stream_));
     if (plan_) fftso3_plan_destroy(plan_);
     if (d_grid_) (void)hipFree(d_grid_);
     if (d_coef_) (void)hipFree(d_coef_);
     if (d_ang_) (void)hipFree(d_ang_);
     if (d_val_) (void)hipFree(d_val_);
+    for (int b = 0; b < 2; b++) {
+        if (pin_[b]) (void)hipHostFree(pin_[b]);
+        if (ev_[b]) (void)hipEventDestroy(static_cast<hipEvent_t>(ev_[b]));
+    }
+    if (stream_) (void)hipStreamDestroy(static_cast<hipStream_t>(stream_));
 }
 
 void DevicePlan::prepare(int domain, int l_max, int n_fn, int kind, int precision)
@@ -134,6 +141,69 @@ void DevicePlan::inverse(int domain, int l_max, int n_fn, int kind, int precisio
     check_abi(fftso3_inverse(plan_, d_coef_, d_grid_, nullptr), "fftso3_inverse");
     check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
     download(d_grid_, grid_elems_ * w, static_cast<double *>(grid), precision, stage_);
+}
+
+void DevicePlan::forward_fill(int domain, int l_max, int kind, int precision, const Fill &fill, void *coef)
+{
+    prepare(domain, l_max, 1, kind, precision);
+    const int w = kind == FFTSO3_COMPLEX ? 2 : 1;      // doubles per grid value
+    const bool fp64 = precision == FFTSO3_FP64;
+    const size_t rsz = fp64 ? 8 : 4;                    // device scalar
+    const long nv = grid_elems_;                        // grid values of the function
+    // chunks of whole 1024-value pieces, about 4 MB of device data each
+    const long piece = 1024;
+    const long chunk = std::max(piece, ((long)(4 << 20) / (long)(w * rsz)) / piece * piece);
+    const size_t bytes = (size_t)std::min(chunk, nv) * w * rsz;
+    if (!stream_) {
+        hipStream_t st;
+        check_hip(hipStreamCreate(&st), "hipStreamCreate");
+        stream_ = st;
+        for (int b = 0; b < 2; b++) {
+            hipEvent_t e;
+            check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            ev_[b] = e;
+        }
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream_);
+    if (bytes > pin_bytes_) {
+        check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int b = 0; b < 2; b++) {
+            if (pin_[b]) (void)hipHostFree(pin_[b]);
+            pin_[b] = nullptr;
+        }
+        pin_bytes_ = 0;
+        for (int b = 0; b < 2; b++) check_hip(hipHostMalloc(&pin_[b], bytes, hipHostMallocDefault), "hipHostMalloc");
+        pin_bytes_ = bytes;
+    }
+    char *dg = static_cast<char *>(d_grid_);
+    long it = 0;
+    for (long c0 = 0; c0 < nv; c0 += chunk, it++) {
+        const int b = (int)(it & 1);
+        if (it >= 2) check_hip(hipEventSynchronize(static_cast<hipEvent_t>(ev_[b])), "hipEventSynchronize");
+        const long n = std::min(chunk, nv - c0), np = (n + piece - 1) / piece;
+        void *buf = pin_[b];
+#pragma omp parallel
+        {
+            std::vector<double> tmp(fp64 ? 0 : (size_t)piece * w);
+#pragma omp for schedule(dynamic, 4)
+            for (long p = 0; p < np; p++) {
+                const long i0 = c0 + p * piece, i1 = std::min(c0 + n, i0 + piece);
+                if (fp64) {
+                    fill(i0, i1, static_cast<double *>(buf) + (i0 - c0) * w);
+                } else {
+                    fill(i0, i1, tmp.data());
+                    float *f = static_cast<float *>(buf) + (i0 - c0) * w;
+                    for (long i = 0; i < (i1 - i0) * w; i++) f[i] = (float)tmp[i];
+                }
+            }
+        }
+        check_hip(hipMemcpyAsync(dg + (size_t)c0 * w * rsz, buf, (size_t)n * w * rsz, hipMemcpyHostToDevice, st),
+                  "hipMemcpyAsync H2D");
+        check_hip(hipEventRecord(static_cast<hipEvent_t>(ev_[b]), st), "hipEventRecord");
+    }
+    check_abi(fftso3_forward(plan_, d_grid_, d_coef_, st), "fftso3_forward");
+    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    download(d_coef_, coef_elems_ * w, static_cast<double *>(coef), precision, stage_);
 }
 
 void DevicePlan::evaluate(int domain, int l_max, int kind, int precision, const void *coef, const double *angles,

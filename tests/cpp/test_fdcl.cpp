@@ -109,7 +109,7 @@ static int cpu_checks()
     return failures;
 }
 
-static double rel(const std::vector<fdcl::complex<double>> &a, const std::vector<fdcl::complex<double>> &b)
+template <typename T> static double rel(const std::vector<T> &a, const std::vector<T> &b)
 {
     double n = 0, d = 0;
     for (size_t i = 0; i < a.size(); i++) {
@@ -219,6 +219,58 @@ static int gpu_checks()
         }
         expect("s2_grid_inverse_vs_pointwise", std::sqrt(err / nrm), 1e-12);
         expect("s2_grid_round_trip", rel(sc.forward_transform(g).v, F.v), 1e-12);
+    }
+    // forward_transform(std::function): the chunked, pipelined host sampling must give exactly the
+    // coefficients of the grid overload on the same samples (l_max = 40: several 4 MB chunks)
+    {
+        const int L = 40, N = 2 * (L + 1);
+        fdcl::FFTSO3_complex fc(L);
+        auto f = [](double a, double b, double g) {
+            const fdcl::Matrix3d R = fdcl::Euler3232R(a, b, g);
+            return fdcl::complex<double>(R(0, 0) + R(1, 1) + R(2, 2), 0.5 * R(0, 2) - R(2, 1));
+        };
+        std::vector<fdcl::complex<double>> g((size_t)N * N * N);
+        for (int k = 0; k < N; k++)
+            for (int j1 = 0; j1 < N; j1++)
+                for (int j2 = 0; j2 < N; j2++)
+                    g[((size_t)k * N + j1) * N + j2] = f(fc.alpha_j(j1), fc.beta_k(k), fc.gamma_j(j2));
+        const auto Fg = fc.forward_transform(g);
+        const auto Fs = fc.forward_transform(std::function<fdcl::complex<double>(double, double, double)>(f));
+        expect("so3_sampled_vs_grid_forward", rel(Fs.v, Fg.v), 1e-15);
+        // tr R = sum_m D^1_mm: only degree 1 is nonzero, F^1 = I / 3 up to the (1,3) term's entries
+        expect("so3_sampled_trR_F1_diag", std::abs(Fs(1, 0, 0) - 1.0 / 3.0), 1e-12);
+        fc.set_precision(FFTSO3_FP32);
+        const auto F32 = fc.forward_transform(std::function<fdcl::complex<double>(double, double, double)>(f));
+        expect("so3_sampled_fp32_vs_fp64", rel(F32.v, Fg.v), 2e-6);
+        fdcl::FFTSO3_real fr(L);
+        auto fre = [](double a, double b, double g) {
+            const fdcl::Matrix3d R = fdcl::Euler3232R(a, b, g);
+            return R(0, 0) + R(1, 1) + R(2, 2) + 0.25 * R(1, 2);
+        };
+        std::vector<double> gr((size_t)N * N * N);
+        for (int k = 0; k < N; k++)
+            for (int j1 = 0; j1 < N; j1++)
+                for (int j2 = 0; j2 < N; j2++)
+                    gr[((size_t)k * N + j1) * N + j2] = fre(fr.alpha_j(j1), fr.beta_k(k), fr.gamma_j(j2));
+        expect("so3_real_sampled_vs_grid_forward",
+               rel(fr.forward_transform(std::function<double(double, double, double)>(fre)).v, fr.forward_transform(gr).v),
+               1e-15);
+        const int L2 = 200, N2 = 2 * (L2 + 1);
+        fdcl::FFTS2_complex sc(L2);
+        auto fs = [](double th, double ph) { return fdcl::complex<double>(std::cos(th) * std::sin(ph), std::sin(th)); };
+        std::vector<fdcl::complex<double>> gs((size_t)N2 * N2);
+        for (int k = 0; k < N2; k++)
+            for (int j = 0; j < N2; j++) gs[(size_t)k * N2 + j] = fs(sc.theta_k(k), sc.phi_j(j));
+        expect("s2_sampled_vs_grid_forward",
+               rel(sc.forward_transform(std::function<fdcl::complex<double>(double, double)>(fs)).v, sc.forward_transform(gs).v),
+               1e-15);
+        fdcl::FFTS2_real sr(L2);
+        auto fsr = [](double th, double ph) { return std::cos(th) + std::sin(th) * std::cos(2 * ph); };
+        std::vector<double> gsr((size_t)N2 * N2);
+        for (int k = 0; k < N2; k++)
+            for (int j = 0; j < N2; j++) gsr[(size_t)k * N2 + j] = fsr(sr.theta_k(k), sr.phi_j(j));
+        expect("s2_real_sampled_vs_grid_forward",
+               rel(sr.forward_transform(std::function<double(double, double)>(fsr)).v, sr.forward_transform(gsr).v), 1e-15);
     }
     return failures;
 }
