@@ -24,6 +24,7 @@
 #include <stdexcept>
 #include <vector>
 
+#include "fdcl_Clebsch_Gordon.hpp"
 #include "fdcl_FFTSO3_matrix.hpp"
 #include "fftso3_amd.h"
 
@@ -145,12 +146,24 @@ public:
     // character chi_l(theta) = sin((2l+1) theta / 2) / sin(theta / 2) (src/fdcl_FFTSO3.cpp:404-415)
     std::vector<double> character(double theta);
 
-    // self checks of the reference (src/fdcl_FFTSO3.cpp:258-415,532-545,1483-1571); each returns an
-    // error measure that is ~0 when the implementation is right
+    // Clebsch-Gordan coefficients C^{l,m}_{l1,m1,l2,m2}: call C.compute(l1, l2), then read C(l,m,l1,m1,l2,m2)
+    // (include/fdcl_FFTSO3.hpp:35-39 of the reference)
+    Clebsch_Gordon_complex C;
+    // derivatives of D^l at the identity along e_1, e_2, e_3: u[i] = d/dt D(exp(t e_i)) at t = 0,
+    // i = 1..3 (u[0] is empty), i.e. -i times the angular momentum matrices
+    // (src/fdcl_FFTSO3.cpp:417-453)
+    std::vector<FFTSO3_matrix_complex> deriv_wigner_D();
+
+    // self checks of the reference (src/fdcl_FFTSO3.cpp:258-415,455-493,532-545,943-986,1483-1571);
+    // each returns an error measure that is ~0 (below 1e-10) when the implementation is right
     void check_all();
     double check_weight();
     double check_wigner_d();
     double check_transform();
+    // D(exp(t e_i)) against the matrix exponential of t u[i] (an exact identity, no finite difference)
+    double check_deriv_wigner_D();
+    // D^{l1}_{m1 n1} D^{l2}_{m2 n2} = sum_l C^{l,m1+m2}_{l1,m1,l2,m2} C^{l,n1+n2}_{l1,n1,l2,n2} D^l at a random rotation
+    double check_Clebsch_Gordon();
 
 #ifdef FDCL_HAVE_EIGEN
     FFTSO3_matrix_complex wigner_D(const Eigen::Matrix3d &R) { return wigner_D(to_matrix3d(R)); }
@@ -218,9 +231,18 @@ public:
     // T^l of src/fdcl_FFTSO3.cpp:1107-1139
     FFTSO3_matrix_complex matrix2rsph(int L);
 
+    // Clebsch-Gordan coefficients of the real harmonics: c.compute(l1, l2), then c(l,m,l1,m1,l2,m2)
+    Clebsch_Gordon_real c;
+    // derivatives of U^l at the identity along e_1, e_2, e_3 (u[0] empty): conj(T) u_D T^T with
+    // the complex derivatives u_D (src/fdcl_FFTSO3.cpp:1577-1645)
+    std::vector<FFTSO3_matrix_real> deriv_real_harmonics();
+
     void check_all();
     double check_transform();
     double check_real_harmonics();
+    double check_deriv_real_harmonics();  // U(exp(t e_i)) against expm(t u[i])
+    // U^{l1}_{m1 n1} U^{l2}_{m2 n2} = sum Re(c c*) U^l at a random rotation (src/fdcl_FFTSO3.cpp:1685-1768)
+    double check_Clebsch_Gordon();
 
 protected:
     FFTSO3_matrix_real real_harmonics(double alpha, double beta, double gamma, int L);

@@ -2,6 +2,7 @@
 // quadrature weights, pointwise inverses and self checks; transforms go to the GPU.
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <iostream>
@@ -295,10 +296,143 @@ double FFTSO3_complex::check_transform()
     return err;
 }
 
+// u[i] = d/dt D(exp(t e_i)) at t = 0 = -i J_i in the |l m> basis: J_z = m, J_x = (J+ + J-)/2,
+// J_y = (J+ - J-)/(2i) with <m+1|J+|m> = sqrt((l-m)(l+m+1)), <m-1|J-|m> = sqrt((l+m)(l-m+1))
+std::vector<FFTSO3_matrix_complex> FFTSO3_complex::deriv_wigner_D()
+{
+    std::vector<FFTSO3_matrix_complex> u(4);
+    for (int i = 1; i <= 3; i++) u[i].init(l_max);
+    for (int l = 0; l <= l_max; l++)
+        for (int n = -l; n <= l; n++) {
+            u[3](l, n, n) = -I * (double)n;
+            if (n < l) {   // <n+1| J+ |n>
+                const double jp = std::sqrt((double)(l - n) * (l + n + 1));
+                u[1](l, n + 1, n) = -0.5 * I * jp;
+                u[2](l, n + 1, n) = -0.5 * jp;
+            }
+            if (n > -l) {  // <n-1| J- |n>
+                const double jm = std::sqrt((double)(l + n) * (l - n + 1));
+                u[1](l, n - 1, n) = -0.5 * I * jm;
+                u[2](l, n - 1, n) = 0.5 * jm;
+            }
+        }
+    return u;
+}
+
+namespace
+{
+// R = exp(hat(r)) by Rodrigues' formula
+Matrix3d expm_so3(const double r[3])
+{
+    const double th = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    const double a = th > 0 ? std::sin(th) / th : 1.0, b = th > 0 ? (1.0 - std::cos(th)) / (th * th) : 0.5;
+    const double K[3][3] = {{0, -r[2], r[1]}, {r[2], 0, -r[0]}, {-r[1], r[0], 0}};
+    Matrix3d R;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double k2 = 0;
+            for (int q = 0; q < 3; q++) k2 += K[i][q] * K[q][j];
+            R(i, j) = (i == j ? 1.0 : 0.0) + a * K[i][j] + b * k2;
+        }
+    return R;
+}
+
+// exp(A) of an n x n column-major matrix: scaling and squaring of a 24-term Taylor sum
+template <typename T> std::vector<T> expm(const std::vector<T> &A, int n)
+{
+    double nrm = 0;
+    for (const T &x : A) nrm = std::max(nrm, std::abs(x));
+    int sq = 0;
+    while (nrm * n > 0.5) {
+        nrm *= 0.5;
+        sq++;
+    }
+    const double sc = std::ldexp(1.0, -sq);
+    std::vector<T> X(A.size()), E(A.size(), T(0)), P(A.size(), T(0)), Q(A.size());
+    for (size_t i = 0; i < A.size(); i++) X[i] = A[i] * sc;
+    for (int i = 0; i < n; i++) E[i * n + i] = P[i * n + i] = T(1);
+    for (int k = 1; k <= 24; k++) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                T t = 0;
+                for (int q = 0; q < n; q++) t += P[q * n + i] * X[j * n + q];
+                Q[j * n + i] = t / (double)k;
+            }
+        P.swap(Q);
+        for (size_t i = 0; i < E.size(); i++) E[i] += P[i];
+    }
+    for (int r = 0; r < sq; r++) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                T t = 0;
+                for (int q = 0; q < n; q++) t += E[q * n + i] * E[j * n + q];
+                Q[j * n + i] = t;
+            }
+        E.swap(Q);
+    }
+    return E;
+}
+}  // namespace
+
+// D is a representation, so D(exp(t e_i)) = exp(t u[i]) exactly; compared at t = 0.4 for every
+// l <= l_max (relative Frobenius error, largest over l and i)
+double FFTSO3_complex::check_deriv_wigner_D()
+{
+    const std::vector<FFTSO3_matrix_complex> u = deriv_wigner_D();
+    const double t = 0.4;
+    double err = 0.0;
+    for (int i = 1; i <= 3; i++) {
+        double r[3] = {0, 0, 0};
+        r[i - 1] = t;
+        const std::vector<double> abg = R2Euler323(expm_so3(r));
+        const FFTSO3_matrix_complex D = wigner_D(abg[0], abg[1], abg[2], l_max);
+        for (int l = 1; l <= l_max; l++) {
+            const int n = 2 * l + 1;
+            std::vector<complex<double>> A(n * n);
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) A[b * n + a] = t * u[i](l, a - l, b - l);
+            const std::vector<complex<double>> E = expm(A, n);
+            double e = 0, d = 0;
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    e += std::norm(E[b * n + a] - D(l, a - l, b - l));
+                    d += std::norm(D(l, a - l, b - l));
+                }
+            err = std::max(err, std::sqrt(e / d));
+        }
+    }
+    std::cout << "fdcl::FFTSO3_complex::check_deriv_wigner_D: error = " << err << std::endl;
+    return err;
+}
+
+double FFTSO3_complex::check_Clebsch_Gordon()
+{
+    const int l1 = 2, l2 = 5;
+    const double a = (double)std::rand() / RAND_MAX * 2 * M_PI, b = (double)std::rand() / RAND_MAX * M_PI,
+                 g = (double)std::rand() / RAND_MAX * 2 * M_PI;
+    const FFTSO3_matrix_complex D = wigner_D(a, b, g, l1 + l2);
+    C.compute(l1, l2);
+    double err = 0.0;
+    for (int m1 = -l1; m1 <= l1; m1++)
+        for (int n1 = -l1; n1 <= l1; n1++)
+            for (int m2 = -l2; m2 <= l2; m2++)
+                for (int n2 = -l2; n2 <= l2; n2++) {
+                    complex<double> y = 0.0;
+                    const int lo = std::max(std::abs(l1 - l2), std::max(std::abs(m1 + m2), std::abs(n1 + n2)));
+                    for (int l = lo; l <= l1 + l2; l++)
+                        y += C(l, m1 + m2, l1, m1, l2, m2) * C(l, n1 + n2, l1, n1, l2, n2) * D(l, m1 + m2, n1 + n2);
+                    err = std::max(err, std::abs(D(l1, m1, n1) * D(l2, m2, n2) - y));
+                }
+    std::cout << "fdcl::FFTSO3_complex::check_Clebsch_Gordon: error = " << err << std::endl;
+    return err;
+}
+
 void FFTSO3_complex::check_all()
 {
     check_weight();
     check_wigner_d();
+    check_deriv_wigner_D();
+    check_Clebsch_Gordon();
     check_transform();
 }
 
@@ -487,11 +621,94 @@ double FFTSO3_real::check_transform()
     return err;
 }
 
+std::vector<FFTSO3_matrix_real> FFTSO3_real::deriv_real_harmonics()
+{
+    const std::vector<FFTSO3_matrix_complex> uc = deriv_wigner_D();
+    const FFTSO3_matrix_complex T = matrix2rsph(l_max);
+    std::vector<FFTSO3_matrix_real> u(4);
+    for (int i = 1; i <= 3; i++) {
+        u[i].init(l_max);
+        for (int l = 0; l <= l_max; l++)
+            for (int m = -l; m <= l; m++)
+                for (int n = -l; n <= l; n++) {
+                    complex<double> s = 0.0;
+                    for (int p = -l; p <= l; p++) {
+                        if (T(l, m, p) == 0.0) continue;
+                        for (int q = -l; q <= l; q++)
+                            if (T(l, n, q) != 0.0) s += std::conj(T(l, m, p)) * uc[i](l, p, q) * T(l, n, q);
+                    }
+                    u[i](l, m, n) = s.real();   // the imaginary part vanishes
+                }
+    }
+    return u;
+}
+
+double FFTSO3_real::check_deriv_real_harmonics()
+{
+    const std::vector<FFTSO3_matrix_real> u = deriv_real_harmonics();
+    const double t = 0.4;
+    double err = 0.0;
+    for (int i = 1; i <= 3; i++) {
+        double r[3] = {0, 0, 0};
+        r[i - 1] = t;
+        const std::vector<double> abg = R2Euler323(expm_so3(r));
+        const FFTSO3_matrix_real U = real_harmonics(abg[0], abg[1], abg[2], l_max);
+        for (int l = 1; l <= l_max; l++) {
+            const int n = 2 * l + 1;
+            std::vector<double> A(n * n);
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) A[b * n + a] = t * u[i](l, a - l, b - l);
+            const std::vector<double> E = expm(A, n);
+            double e = 0, d = 0;
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    e += std::pow(E[b * n + a] - U(l, a - l, b - l), 2);
+                    d += std::pow(U(l, a - l, b - l), 2);
+                }
+            err = std::max(err, std::sqrt(e / d));
+        }
+    }
+    std::cout << "fdcl::FFTSO3_real::check_deriv_real_harmonics: error = " << err << std::endl;
+    return err;
+}
+
+double FFTSO3_real::check_Clebsch_Gordon()
+{
+    const int l1 = 3, l2 = 5;
+    const double a = (double)std::rand() / RAND_MAX * 2 * M_PI, b = (double)std::rand() / RAND_MAX * M_PI,
+                 g = (double)std::rand() / RAND_MAX * 2 * M_PI;
+    const FFTSO3_matrix_real U = real_harmonics(a, b, g, l1 + l2);
+    c.compute(l1, l2);
+    double err = 0.0;
+    for (int m1 = -l1; m1 <= l1; m1++)
+        for (int n1 = -l1; n1 <= l1; n1++)
+            for (int m2 = -l2; m2 <= l2; m2++)
+                for (int n2 = -l2; n2 <= l2; n2++) {
+                    // the real harmonics of order m1 and m2 couple to orders +-m1 +-m2 only
+                    int M[4] = {m1 + m2, m1 - m2, -m1 + m2, -m1 - m2}, Nn[4] = {n1 + n2, n1 - n2, -n1 + n2, -n1 - n2};
+                    std::sort(M, M + 4);
+                    std::sort(Nn, Nn + 4);
+                    const int nm = (int)(std::unique(M, M + 4) - M), nn = (int)(std::unique(Nn, Nn + 4) - Nn);
+                    double y = 0.0;
+                    for (int x = 0; x < nm; x++)
+                        for (int z = 0; z < nn; z++)
+                            for (int l = std::max(std::abs(l1 - l2), std::max(std::abs(M[x]), std::abs(Nn[z])));
+                                 l <= l1 + l2; l++)
+                                y += std::real(c(l, M[x], l1, m1, l2, m2) * std::conj(c(l, Nn[z], l1, n1, l2, n2))) *
+                                     U(l, M[x], Nn[z]);
+                    err = std::max(err, std::fabs(U(l1, m1, n1) * U(l2, m2, n2) - y));
+                }
+    std::cout << "fdcl::FFTSO3_real::check_Clebsch_Gordon: error = " << err << std::endl;
+    return err;
+}
+
 void FFTSO3_real::check_all()
 {
     check_weight();
     check_wigner_d();
     check_real_harmonics();
+    check_Clebsch_Gordon();
+    check_deriv_real_harmonics();
     check_transform();
 }
 

@@ -61,6 +61,34 @@ static int cpu_checks()
         fdcl::FFTS2_complex sc(L);
         expect(("s2_check_weight_l" + std::to_string(L)).c_str(), sc.check_weight(), 1e-12);
     }
+    // the reference's remaining self checks (test/test.cpp:53,57,66,70 hold them below 1e-10)
+    for (int L : {5, 20}) {
+        fdcl::FFTSO3_complex fc(L);
+        expect(("so3_check_deriv_wigner_D_l" + std::to_string(L)).c_str(), fc.check_deriv_wigner_D(), 1e-10);
+        fdcl::FFTSO3_real fr(L);
+        expect(("so3_check_deriv_real_harmonics_l" + std::to_string(L)).c_str(), fr.check_deriv_real_harmonics(), 1e-10);
+    }
+    {
+        fdcl::FFTSO3_complex fc(5);
+        expect("so3_check_Clebsch_Gordon", fc.check_Clebsch_Gordon(), 1e-10);
+        fdcl::FFTSO3_real fr(5);
+        expect("so3_check_Clebsch_Gordon_real", fr.check_Clebsch_Gordon(), 1e-10);
+        // real derivative matrices are real: conj(T) u T^T has no imaginary part; spot values of u_3
+        const auto u = fc.deriv_wigner_D();
+        expect("so3_deriv_u3_value", std::abs(u[3](4, 3, 3) - fdcl::complex<double>(0, -3)), 0);
+        // CG orthogonality: the (l1,l2) block is an orthogonal matrix
+        fdcl::Clebsch_Gordon_complex cg(3, 4);
+        cg.compute(3, 4);
+        const int n = 7 * 9;
+        double eo = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                double t = 0;
+                for (int q = 0; q < n; q++) t += cg.C(q, i) * cg.C(q, j);
+                eo = std::max(eo, std::fabs(t - (i == j ? 1.0 : 0.0)));
+            }
+        expect("cg_block_orthogonal", eo, 1e-12);
+    }
     // d at beta and -beta: d^l_mn(-beta) = (-1)^(m-n) d^l_mn(beta); wigner_D phases
     {
         fdcl::FFTSO3_complex fc(9);
@@ -281,6 +309,18 @@ int main(int argc, char **argv)
     try {
         if (mode == "cpu") return cpu_checks() ? 1 : 0;
         if (mode == "gpu") return gpu_checks() ? 1 : 0;
+        if (mode == "cg" && argc > 3) {   // C^{l,m}_{l1,m1,l2,m2} for every (l, m, m1): "l m m1 m2 value"
+            const int l1 = std::atoi(argv[2]), l2 = std::atoi(argv[3]);
+            fdcl::Clebsch_Gordon_complex cg;
+            cg.compute(l1, l2);
+            for (int l = std::abs(l1 - l2); l <= l1 + l2; l++)
+                for (int m = -l; m <= l; m++)
+                    for (int m1 = -l1; m1 <= l1; m1++) {
+                        const int m2 = m - m1;
+                        if (std::abs(m2) <= l2) std::printf("%d %d %d %d %.17g\n", l, m, m1, m2, cg(l, m, l1, m1, l2, m2));
+                    }
+            return 0;
+        }
         if (mode == "dump" && argc > 3) {
             fdcl::FFTSO3_complex fc(std::atoi(argv[3]));
             const auto d = fc.wigner_d(std::atof(argv[2]));

@@ -51,6 +51,25 @@ def test_class_layer_wigner_d_vs_oracle(exe, oracle, beta, L):
     assert np.max(np.abs(d - ref)) < 1e-12
 
 
+@pytest.mark.parametrize("l1,l2", [(1, 1), (2, 5), (3, 3), (6, 4), (0, 3)])
+def test_clebsch_gordan_vs_sympy(exe, l1, l2):
+    """Clebsch-Gordan coefficients of the class layer against sympy's exact CG (Condon-Shortley
+    phase): every coefficient of the (l1, l2) block, zeros included."""
+    from sympy import S
+    from sympy.physics.quantum.cg import CG
+    rc, out = _run(exe, "cg", l1, l2)
+    assert rc == 0, out
+    rows = [line.split() for line in out.strip().splitlines()]
+    expected = sum(1 for l in range(abs(l1 - l2), l1 + l2 + 1) for m in range(-l, l + 1)
+                   for m1 in range(-l1, l1 + 1) if abs(m - m1) <= l2)
+    assert len(rows) == expected
+    err = 0.0
+    for l, m, m1, m2, v in rows:
+        ref = float(CG(S(l1), S(m1), S(l2), S(m2), S(l), S(m)).doit())
+        err = max(err, abs(float(v) - ref))
+    assert err < 1e-13
+
+
 @pytest.mark.parametrize("x,L", [(0.37, 20), (-0.93, 64), (0.0, 15)])
 def test_class_layer_legendre_vs_oracle(exe, oracle, x, L):
     rc, out = _run(exe, "dumpnp", x, L)
