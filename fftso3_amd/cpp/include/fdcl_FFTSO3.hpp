@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "fdcl_Clebsch_Gordon.hpp"
+#include "fdcl_FFTS2_matrix.hpp"
 #include "fdcl_FFTSO3_matrix.hpp"
 #include "fftso3_amd.h"
 
@@ -233,6 +234,14 @@ public:
 
     // Clebsch-Gordan coefficients of the real harmonics: c.compute(l1, l2), then c(l,m,l1,m1,l2,m2)
     Clebsch_Gordon_real c;
+
+    // S^2 workflows of example/example3.cpp:50-53 and example/example3_SSM.hpp:62-70:
+    // real spherical-harmonic coefficients of a rotated function, G^l = U^l(R) F^l
+    FFTS2_matrix_real rotate(const FFTS2_matrix_real &F, const Matrix3d &R);
+    // the matching cost J(R) = sum_l (G^l)^T U^l(R) F^l at every rotation of the SO(3) grid
+    // [k][j1][j2] = (alpha_j1, beta_k, gamma_j2), l <= l_max: one real inverse transform on the GPU
+    // of H^l_mn = G^l_m F^l_n / (2l+1)
+    std::vector<double> correlation(const FFTS2_matrix_real &F, const FFTS2_matrix_real &G);
     // derivatives of U^l at the identity along e_1, e_2, e_3 (u[0] empty): conj(T) u_D T^T with
     // the complex derivatives u_D (src/fdcl_FFTSO3.cpp:1577-1645)
     std::vector<FFTSO3_matrix_real> deriv_real_harmonics();

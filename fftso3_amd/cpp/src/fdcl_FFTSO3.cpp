@@ -621,6 +621,30 @@ double FFTSO3_real::check_transform()
     return err;
 }
 
+FFTS2_matrix_real FFTSO3_real::rotate(const FFTS2_matrix_real &F, const Matrix3d &R)
+{
+    const std::vector<double> abg = R2Euler323(R);
+    const FFTSO3_matrix_real U = real_harmonics(abg[0], abg[1], abg[2], F.l_max);
+    FFTS2_matrix_real G(F.l_max);
+    for (int l = 0; l <= F.l_max; l++)
+        for (int m = -l; m <= l; m++) {
+            double s = 0.0;
+            for (int n = -l; n <= l; n++) s += U(l, m, n) * F(l, n);
+            G(l, m) = s;
+        }
+    return G;
+}
+
+std::vector<double> FFTSO3_real::correlation(const FFTS2_matrix_real &F, const FFTS2_matrix_real &G)
+{
+    const int L = l_max;
+    FFTSO3_matrix_real H(L);
+    for (int l = 0; l <= std::min(L, std::min(F.l_max, G.l_max)); l++)
+        for (int m = -l; m <= l; m++)
+            for (int n = -l; n <= l; n++) H(l, m, n) = G(l, m) * F(l, n) / (2.0 * l + 1.0);
+    return inverse_transform(H);
+}
+
 std::vector<FFTSO3_matrix_real> FFTSO3_real::deriv_real_harmonics()
 {
     const std::vector<FFTSO3_matrix_complex> uc = deriv_wigner_D();

@@ -7,6 +7,7 @@
 //   test_fdcl dumpnp X L  print nP^l_m(X) for l <= L, packed order
 //
 // Exit status 0 iff every check passes; one line per check on stdout.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -247,6 +248,32 @@ static int gpu_checks()
         }
         expect("s2_grid_inverse_vs_pointwise", std::sqrt(err / nrm), 1e-12);
         expect("s2_grid_round_trip", rel(sc.forward_transform(g).v, F.v), 1e-12);
+    }
+    // S^2 matching (example/example3.cpp): the correlation grid from one GPU inverse equals the cost
+    // J(R) = sum_l G^T U(R) F at grid rotations, and peaks at the rotation applied to F
+    {
+        const int L = 16, N = 2 * (L + 1);
+        fdcl::FFTSO3_real fr(L);
+        fdcl::FFTS2_matrix_real F(L);
+        F.setRandom();
+        const int k0 = 9, a0 = 5, g0 = 21;
+        const fdcl::Matrix3d Rt = fdcl::Euler3232R(fr.alpha_j(a0), fr.beta_k(k0), fr.gamma_j(g0));
+        const fdcl::FFTS2_matrix_real G = fr.rotate(F, Rt);
+        const std::vector<double> J = fr.correlation(F, G);
+        double err = 0, nrm = 0;
+        for (int s = 0; s < 20; s++) {
+            const int k = (7 * s + 2) % N, j1 = (11 * s + 1) % N, j2 = (5 * s + 3) % N;
+            const fdcl::FFTSO3_matrix_real U = fr.real_harmonics(fr.alpha_j(j1), fr.beta_k(k), fr.gamma_j(j2));
+            double ref = 0;
+            for (int l = 0; l <= L; l++)
+                for (int m = -l; m <= l; m++)
+                    for (int n = -l; n <= l; n++) ref += G(l, m) * U(l, m, n) * F(l, n);
+            err += std::pow(J[((size_t)k * N + j1) * N + j2] - ref, 2);
+            nrm += ref * ref;
+        }
+        expect("s2_correlation_vs_direct_cost", std::sqrt(err / nrm), 1e-10);
+        const size_t best = std::max_element(J.begin(), J.end()) - J.begin();
+        expect("s2_correlation_peak_at_true_rotation", (double)(best != ((size_t)k0 * N + a0) * N + g0), 0);
     }
     // forward_transform(std::function): the chunked, pipelined host sampling must give exactly the
     // coefficients of the grid overload on the same samples (l_max = 40: several 4 MB chunks)
