@@ -29,6 +29,7 @@ _SIGS = {
     "fftso3_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fftso3_plan_set_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_plan_get_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "fftso3_plan_describe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     "fftso3_plan_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
                                        ctypes.c_void_p, ctypes.c_void_p]),
@@ -212,6 +213,12 @@ class Plan:
     def set_graph(self, on=True):
         """Capture each (direction, input, output, stream) into a hipGraph and replay it."""
         _check(lib().fftso3_plan_set_graph(self._h, 1 if on else 0))
+
+    def describe(self):
+        """The current schedule, one line per bandlimit class."""
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().fftso3_plan_describe(self._h, buf, len(buf)))
+        return buf.value.decode()
 
     def autotune(self, grid=None, coef=None, stream=None, reps=2):
         """Pick the fastest schedule for this plan; grid / coef (device tensors) are clobbered."""

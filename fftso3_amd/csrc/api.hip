@@ -1087,6 +1087,25 @@ int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk)
     return finalize_plan(plan);
 }
 
+int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len)
+{
+    if (!plan || !buf || len < 1) return fail(FFTSO3_EINVAL, "NULL argument or empty buffer");
+    std::string out;
+    const char *prec = plan->prec == FFTSO3_FP64 ? "fp64" : plan->prec == FFTSO3_FP32 ? "fp32"
+                     : plan->prec == FFTSO3_BF16X6 ? "bf16x6" : "bf16x3";
+    for (const Class *c : plan->classes) {
+        const char *path = c->mxk ? "batch-wide mfma (k-chunked)" : c->mx ? "batch-wide mfma" : "chunked";
+        char line[256];
+        ChunkArgs ca = chunk_args<float>(plan, const_cast<Class *>(c), 0, (int)c->fns.size());
+        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s\n", c->B - 1,
+                      (int)c->fns.size(), prec, path, c->chunk, c->mx ? plan->klo : 0, ca.gil, plan->split_mul,
+                      c->d_dtab.p ? " d-table" : "", plan->graph ? " graph" : "");
+        out += line;
+    }
+    std::snprintf(buf, (size_t)len, "%s", out.c_str());
+    return FFTSO3_OK;
+}
+
 int fftso3_plan_get_chunk(fftso3_plan_t plan, int *chunk)
 {
     if (!plan || !chunk) return fail(FFTSO3_EINVAL, "NULL argument");

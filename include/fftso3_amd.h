@@ -97,6 +97,11 @@ int fftso3_plan_set_graph(fftso3_plan_t plan, int on);
  * are undefined afterwards.  No counterpart in the reference. */
 int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps);
 
+/* The plan's current schedule as text, one line per bandlimit class (l_max, functions, path,
+ * functions per pass, workspace row blocking and group interleave, group split), written to buf
+ * (at most len bytes, NUL-terminated).  No counterpart in the reference. */
+int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len);
+
 /* Element offsets of function i in the packed grid / coefficient arrays, and the totals. */
 long fftso3_plan_grid_offset(fftso3_plan_t plan, int fn);
 long fftso3_plan_coef_offset(fftso3_plan_t plan, int fn);
