@@ -506,6 +506,15 @@ int persistent_grid(K kernel, size_t lds, int nunits, int threads = kFftThreads)
 bool g_disable_fft128 = false;
 
 template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128 && !g_disable_fft128; }
+// register-based ring FFTs of the S^2 grid at l_max = 127 (FFTSO3_S2FFT256=0 selects the generic kernels)
+template <typename T> bool use_s2fft256(const Class *c)
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_S2FFT256");
+        return e ? atoi(e) : 1;
+    }();
+    return sizeof(T) == 4 && c->N == 256 && on != 0;
+}
 
 // Group interleave of the workspace (ws_rec) for the N = 128 FFT kernels with the batch-wide FP32
 // MFMA contraction and unblocked rows; 0 everywhere else.  Two groups side by side (gil = 1) let
@@ -783,7 +792,13 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         {
             StageTimer t(pl, s, 0);
             FftArgs<T> a = fft_args<T>(pl, c, ca, in, pl->d_ws.p);
-            if (s2) {
+            if (s2 && use_s2fft256<T>(c)) {
+                if constexpr (sizeof(T) == 4) {
+                    const int g256 = xcd_grid(((nfc + S2F_FNS - 1) / S2F_FNS) * (256 / S2F_KS));
+                    if (real) hipLaunchKernelGGL((s2_fft256_fwd<true>), dim3(g256), dim3(256), 0, s, a);
+                    else hipLaunchKernelGGL((s2_fft256_fwd<false>), dim3(g256), dim3(256), 0, s, a);
+                }
+            } else if (s2) {
                 a.tile = c->row_tile;
                 int nunit = nfc * (N / a.tile);
                 size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
@@ -862,7 +877,13 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         {
             StageTimer t(pl, s, 5);
             FftArgs<T> a = fft_args<T>(pl, c, ca, pl->d_ws.p, out);
-            if (s2) {
+            if (s2 && use_s2fft256<T>(c)) {
+                if constexpr (sizeof(T) == 4) {
+                    const int g256 = xcd_grid(((nfc + S2F_FNS - 1) / S2F_FNS) * (256 / S2F_KS));
+                    if (real) hipLaunchKernelGGL((s2_fft256_inv<true>), dim3(g256), dim3(256), 0, s, a);
+                    else hipLaunchKernelGGL((s2_fft256_inv<false>), dim3(g256), dim3(256), 0, s, a);
+                }
+            } else if (s2) {
                 a.tile = c->row_tile;
                 int nunit = nfc * (N / a.tile);
                 size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;

@@ -251,3 +251,126 @@ __global__ void __launch_bounds__(1024) so3_fft128_inv(FftArgs<float> a)
         cols_pass_out(buf, twl, tid, [&](int row, int col, float2 v) { out[row * F128 + col] = v; });
     }
 }
+
+// ---------------------------------------------------------------------------------------------
+// S^2, N = 256 (l_max = 127), FP32: the length-256 DFT of each ring as 16 x 16.  A workgroup of
+// 256 threads takes 16 rings: 2 beta rows k of 8 consecutive functions, 16 threads per ring, so
+// the DFT's exchanges stay inside a wave.  Stage A: thread j holds x[j + 16t] (t < 16), a
+// 16-point DFT in registers and the twiddle W_256^(j s); stage B: thread s takes the 16 values
+// a_s(j) from LDS (row pitch 17: no bank conflicts), a 16-point DFT, and holds X[s + 16q].
+// The 8 functions of a workgroup own whole 128-byte lines of every workspace row, so the
+// spectrum moves as full lines: the forward stages X in LDS and writes (X[g], X[256 - g])
+// records 8 functions at a time; the inverse reads records the same way into LDS.
+// ---------------------------------------------------------------------------------------------
+#define S2F_ROWS 16
+#define S2F_PITCH 17
+#define S2F_FNS 8                      // functions per workgroup
+#define S2F_KS (S2F_ROWS / S2F_FNS)    // beta rows per workgroup
+
+// stage A from registers v (x[j + 16t]), then stage B: v[q] = X[s + 16q] with s = j
+__device__ __forceinline__ void s2f_256(float2 *v, float2 *rowbuf, const float2 *twl, int j)
+{
+    dft16(v);
+#pragma unroll
+    for (int s = 1; s < 16; s++) v[s] = cmul(v[s], twl[(j * s) & 255]);
+#pragma unroll
+    for (int s = 0; s < 16; s++) rowbuf[s * S2F_PITCH + j] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = rowbuf[j * S2F_PITCH + q];
+    dft16(v);
+}
+
+// workgroup -> (first beta row k0, first function f0); consecutive workgroups: consecutive functions
+__device__ __forceinline__ bool s2f_unit(const ChunkArgs &ca, int &k0, int &f0)
+{
+    const int nfb = (ca.nfc + S2F_FNS - 1) / S2F_FNS, nunit = nfb * (256 / S2F_KS);
+    const int unit = xcd_unit(nunit);
+    if (unit >= nunit) return false;
+    f0 = (unit % nfb) * S2F_FNS;
+    k0 = (unit / nfb) * S2F_KS;
+    return true;
+}
+
+template <bool REAL>
+__global__ void __launch_bounds__(256) s2_fft256_fwd(FftArgs<float> a)
+{
+    __shared__ float2 twl[256];
+    __shared__ float2 buf[S2F_ROWS * 16 * S2F_PITCH];
+    const ChunkArgs &ca = a.ca;
+    int k0, f0;
+    if (!s2f_unit(ca, k0, f0)) return;
+    const int tid = threadIdx.x, rr = tid >> 4, j = tid & 15;
+    const int k = k0 + rr / S2F_FNS, fi = f0 + rr % S2F_FNS;   // this thread's ring
+    const bool live = fi < ca.nfc;
+    twl[tid] = a.tw[tid];
+    const long base = live ? ca.grid_off[ca.fn[fi]] + (long)k * 256 : 0;
+    float2 v[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        if (!live) v[t] = make_float2(0.f, 0.f);
+        else if (REAL) v[t] = make_float2(reinterpret_cast<const float *>(a.src)[base + j + 16 * t], 0.f);
+        else v[t] = reinterpret_cast<const float2 *>(a.src)[base + j + 16 * t];
+    }
+    __syncthreads();  // twiddle table
+    float2 *rowbuf = buf + rr * 16 * S2F_PITCH;
+    s2f_256(v, rowbuf, twl, j);
+    __syncthreads();  // stage B's reads of the row buffers
+#pragma unroll
+    for (int q = 0; q < 16; q++) rowbuf[q * S2F_PITCH + j] = v[q];   // X[j + 16q]
+    __syncthreads();
+    // S_k(m) = X(m mod 256), |m| <= 127 (src/fdcl_FFTS2.cpp:159-166): group g = |m| holds the
+    // record (X[g], X[256 - g]); lanes 8i..8i+7 write the 8 functions of one (k, g): a whole line
+    float4 *S4 = reinterpret_cast<float4 *>(a.dst);
+    const int fl = tid % S2F_FNS;
+#pragma unroll
+    for (int i = 0; i < S2F_KS * 128 / 32; i++) {
+        const int idx = tid / S2F_FNS + 32 * i, kl = idx >> 7, g = idx & 127, h = (256 - g) & 255;
+        const float2 *rb = buf + (kl * S2F_FNS + fl) * 16 * S2F_PITCH;
+        const float2 x0 = rb[(g >> 4) * S2F_PITCH + (g & 15)], x1 = rb[(h >> 4) * S2F_PITCH + (h & 15)];
+        if (f0 + fl < ca.nfc)
+            S4[ws_at(ca, g, k0 + kl, 0, f0 + fl, 2) >> 1] = make_float4(x0.x, x0.y, x1.x, x1.y);
+    }
+}
+
+template <bool REAL>
+__global__ void __launch_bounds__(256) s2_fft256_inv(FftArgs<float> a)
+{
+    __shared__ float2 twl[256];
+    __shared__ float2 buf[S2F_ROWS * 16 * S2F_PITCH];
+    const ChunkArgs &ca = a.ca;
+    int k0, f0;
+    if (!s2f_unit(ca, k0, f0)) return;
+    const int tid = threadIdx.x, rr = tid >> 4, j = tid & 15;
+    twl[tid] = a.tw[tid];
+    // records (G(g), G(-g)) of the 8 functions of each (k, g), whole lines; bin u of a ring holds
+    // G(m) with -m = u (mod 256): G(g) at bin (256 - g) & 255, G(-g) at bin g; bin 128 is empty
+    const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
+    const int fl = tid % S2F_FNS;
+#pragma unroll
+    for (int i = 0; i < S2F_KS * 128 / 32; i++) {
+        const int idx = tid / S2F_FNS + 32 * i, kl = idx >> 7, g = idx & 127;
+        const float4 r = f0 + fl < ca.nfc ? G4[ws_at(ca, g, k0 + kl, 0, f0 + fl, 2) >> 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 *rb = buf + (kl * S2F_FNS + fl) * 16 * S2F_PITCH;
+        const int u0 = (256 - g) & 255;
+        rb[(u0 >> 4) * S2F_PITCH + (u0 & 15)] = make_float2(r.x, r.y);
+        if (g > 0) rb[(g >> 4) * S2F_PITCH + (g & 15)] = make_float2(r.z, r.w);
+    }
+    if (tid < S2F_ROWS) buf[tid * 16 * S2F_PITCH + 8 * S2F_PITCH] = make_float2(0.f, 0.f);   // bin 128
+    __syncthreads();
+    float2 *rowbuf = buf + rr * 16 * S2F_PITCH;
+    float2 v[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) v[t] = rowbuf[t * S2F_PITCH + j];   // bin j + 16t
+    __syncthreads();  // every ring's bins read before stage A overwrites the buffers
+    s2f_256(v, rowbuf, twl, j);
+    const int k = k0 + rr / S2F_FNS, fi = f0 + rr % S2F_FNS;
+    if (fi < ca.nfc) {
+        const long base = ca.grid_off[ca.fn[fi]] + (long)k * 256;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (REAL) reinterpret_cast<float *>(a.dst)[base + j + 16 * q] = v[q].x;
+            else reinterpret_cast<float2 *>(a.dst)[base + j + 16 * q] = v[q];
+        }
+    }
+}

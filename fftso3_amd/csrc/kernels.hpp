@@ -340,7 +340,8 @@ __global__ void __launch_bounds__(1024) s2_fft_fwd(FftArgs<T> a)
     int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
-    const int rb = unit % nblk, fi = unit / nblk;
+    // consecutive units: the same rows of consecutive functions, whose workspace records share lines
+    const int fi = unit % ca.nfc, rb = unit / ca.nfc;
     const int fn = ca.fn[fi];
     const int tid = threadIdx.x, nthr = blockDim.x;
     load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
@@ -377,7 +378,8 @@ __global__ void __launch_bounds__(1024) s2_fft_inv(FftArgs<T> a)
     int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
-    const int rb = unit % nblk, fi = unit / nblk;
+    // consecutive units: the same rows of consecutive functions, whose workspace records share lines
+    const int fi = unit % ca.nfc, rb = unit / ca.nfc;
     const int fn = ca.fn[fi];
     const int tid = threadIdx.x, nthr = blockDim.x;
     load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);

@@ -208,7 +208,7 @@ def test_s2_split_bf16_b128(fftso3, torch_cuda, oracle, prec):
         assert rel_l2(c[i], F[i]) < SPLIT_TOL[prec]
 
 
-@pytest.mark.parametrize("domain,B", [("s2", 75), ("s2", 100), ("so3", 70), ("so3", 100), ("so3", 128)])
+@pytest.mark.parametrize("domain,B", [("s2", 75), ("s2", 100), ("s2", 128), ("so3", 70), ("so3", 100), ("so3", 128)])
 def test_batchwide_large_n_fp32(fftso3, torch_cuda, oracle, domain, B):
     """FP32 batch-wide contraction with the d block resident in LDS (128 < N <= 256), including N
     that are not multiples of 32; 3 functions (S^2: 5) against the oracle, complex and real."""
