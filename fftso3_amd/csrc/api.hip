@@ -506,6 +506,17 @@ int persistent_grid(K kernel, size_t lds, int nunits, int threads = kFftThreads)
 bool g_disable_fft128 = false;
 
 template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128 && !g_disable_fft128; }
+// N = 128 forward FFT that takes the two functions sharing each workspace line (gil = 1;
+// FFTSO3_FFT128_PAIRS=0 selects one slice per workgroup)
+static bool fft128_pairs()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FFT128_PAIRS");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 // register-based ring FFTs of the S^2 grid at l_max = 127 (FFTSO3_S2FFT256=0 selects the generic kernels)
 template <typename T> bool use_s2fft256(const Class *c)
 {
@@ -807,7 +818,11 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             } else if (use_fft128<T>(c)) {
                 if constexpr (sizeof(T) == 4) {
                     size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
-                    if (real) hipLaunchKernelGGL((so3_fft128_fwd<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                    if (ca.gil == 1 && fft128_pairs()) {
+                        const int g2 = xcd_grid((nfc + 1) / 2 * N);
+                        if (real) hipLaunchKernelGGL((so3_fft128_fwd2<true>), dim3(g2), dim3(1024), lds, s, a);
+                        else hipLaunchKernelGGL((so3_fft128_fwd2<false>), dim3(g2), dim3(1024), lds, s, a);
+                    } else if (real) hipLaunchKernelGGL((so3_fft128_fwd<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                     else hipLaunchKernelGGL((so3_fft128_fwd<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                 }
             } else if (c->fused) {
@@ -998,6 +1013,8 @@ int set_kernel_attrs()
     ALLOW_LDS((so3_fft128_fwd<false>));
     ALLOW_LDS((so3_fft128_inv<true>));
     ALLOW_LDS((so3_fft128_inv<false>));
+    ALLOW_LDS((so3_fft128_fwd2<true>));
+    ALLOW_LDS((so3_fft128_fwd2<false>));
     ALLOW_LDS((contract_fwd_bf16<4, 2>));
     ALLOW_LDS((contract_fwd_bf16<4, 3>));
     ALLOW_LDS((contract_fwd_bf16<2, 2>));
