@@ -517,6 +517,15 @@ static bool fft128_pairs()
     return on != 0;
 }
 
+static bool fft128_pairs_inv()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FFT128_PAIRS_INV");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 // register-based ring FFTs of the S^2 grid at l_max = 127 (FFTSO3_S2FFT256=0 selects the generic kernels)
 template <typename T> bool use_s2fft256(const Class *c)
 {
@@ -907,7 +916,11 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             } else if (use_fft128<T>(c)) {
                 if constexpr (sizeof(T) == 4) {
                     size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
-                    if (real) hipLaunchKernelGGL((so3_fft128_inv<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                    if (ca.gil == 1 && fft128_pairs_inv()) {
+                        const int g2 = xcd_grid((nfc + 1) / 2 * N);
+                        if (real) hipLaunchKernelGGL((so3_fft128_inv2<true>), dim3(g2), dim3(1024), lds, s, a);
+                        else hipLaunchKernelGGL((so3_fft128_inv2<false>), dim3(g2), dim3(1024), lds, s, a);
+                    } else if (real) hipLaunchKernelGGL((so3_fft128_inv<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                     else hipLaunchKernelGGL((so3_fft128_inv<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                 }
             } else if (c->fused) {
@@ -1015,6 +1028,8 @@ int set_kernel_attrs()
     ALLOW_LDS((so3_fft128_inv<false>));
     ALLOW_LDS((so3_fft128_fwd2<true>));
     ALLOW_LDS((so3_fft128_fwd2<false>));
+    ALLOW_LDS((so3_fft128_inv2<true>));
+    ALLOW_LDS((so3_fft128_inv2<false>));
     ALLOW_LDS((contract_fwd_bf16<4, 2>));
     ALLOW_LDS((contract_fwd_bf16<4, 3>));
     ALLOW_LDS((contract_fwd_bf16<2, 2>));

@@ -379,8 +379,7 @@ __global__ void __launch_bounds__(256) s2_fft256_inv(FftArgs<float> a)
 // SO(3), N = 128 forward, two functions per workgroup (workspace with 2 groups interleaved,
 // gil = 1): the two slices (k, fi) and (k, fi + 1) share every 128-byte workspace line (64 bytes
 // each), so one workgroup transforms both in turn, keeps the first slice's group records in
-// registers until the second's are ready, and writes each line whole.  (The same pairing for the
-// inverse, gathering both slices' records at once, measured slower: 3.0 vs 2.1 ms.)
+// registers until the second's are ready, and writes each line whole.
 // ---------------------------------------------------------------------------------------------
 
 // 2-D forward DFT of slice (k, fi) into LDS (the passes of so3_fft128_fwd)
@@ -470,4 +469,75 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
         S4[o + 4] = h0;
         S4[o + 5] = h1;
     }
+}
+
+// Inverse with the same pairing: both slices' records are gathered at once (whole lines), then
+// each slice is scattered to bins, transformed and written in turn.
+template <bool REAL>
+__device__ __forceinline__ void fft128_inv_slice(const FftArgs<float> &a, float2 *buf, const float2 *twl,
+                                                 const float4 (&q)[4][2], int k, int fi, int tid)
+{
+    const ChunkArgs &ca = a.ca;
+    if (tid < F128) {
+        buf[64 * P128 + tid] = make_float2(0.f, 0.f);
+        buf[tid * P128 + 64] = make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int g = tid + i * 1024;
+        int m0, n0;
+        so3_group(g, m0, n0);
+        const int vm = m0 & 127, vn = n0 & 127, um = (-m0) & 127, un = (-n0) & 127;
+        buf[vm * P128 + vn] = make_float2(q[i][0].x, q[i][0].y);
+        if (m0 > 0) {
+            buf[um * P128 + un] = make_float2(q[i][0].z, q[i][0].w);
+            if (n0 != m0 && n0 != -m0) {
+                buf[un * P128 + um] = make_float2(q[i][1].x, q[i][1].y);
+                buf[vn * P128 + vm] = make_float2(q[i][1].z, q[i][1].w);
+            }
+        }
+    }
+    __syncthreads();
+    rows_from_lds(buf, twl, tid);
+    __syncthreads();
+    rows_stageB(buf, tid);
+    __syncthreads();
+    const long base = ca.grid_off[ca.fn[fi]] + (long)k * F128 * F128;
+    if (REAL) {
+        float *out = reinterpret_cast<float *>(a.dst) + base;
+        cols_pass_out(buf, twl, tid, [&](int row, int col, float2 v) { out[row * F128 + col] = v.x; });
+    } else {
+        float2 *out = reinterpret_cast<float2 *>(a.dst) + base;
+        cols_pass_out(buf, twl, tid, [&](int row, int col, float2 v) { out[row * F128 + col] = v; });
+    }
+}
+
+template <bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft128_inv2(FftArgs<float> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *buf = reinterpret_cast<float2 *>(smem_raw);
+    float2 *twl = buf + F128 * P128;
+    const ChunkArgs &ca = a.ca;
+    const int nfp = (ca.nfc + 1) / 2, nunit = nfp * F128;
+    const int unit = xcd_unit(nunit);
+    if (unit >= nunit) return;
+    const int k = unit / nfp, fi = 2 * (unit - k * nfp);
+    const bool two = fi + 1 < ca.nfc;
+    const int tid = threadIdx.x;
+    if (tid < F128) twl[tid] = a.tw[tid];
+    float4 q0[4][2], q1[4][2];
+    const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const long o = ws_rec(ca, tid + i * 1024, k, fi) * 2, o1 = two ? o + 4 : o;   // select the address, not the value
+        q0[i][0] = G4[o];
+        q0[i][1] = G4[o + 1];
+        q1[i][0] = G4[o1];
+        q1[i][1] = G4[o1 + 1];
+    }
+    fft128_inv_slice<REAL>(a, buf, twl, q0, k, fi, tid);
+    if (!two) return;
+    __syncthreads();  // the first slice's LDS reads before the second slice's scatter
+    fft128_inv_slice<REAL>(a, buf, twl, q1, k, fi + 1, tid);
 }
