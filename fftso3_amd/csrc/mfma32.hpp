@@ -766,12 +766,18 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
             // the next row (or the index table): finite or not, they only reach rows k >= N of
             // the result, which are never stored
             const float *Dt = D + kh * DP + 32 * t + cl;
+            // 16 operand reads from LDS ahead of their 16 MFMAs, so no MFMA waits on its own read
 #pragma unroll
-            for (int s = 0; s < 64; s += 2) {
-                if (2 * s < M4)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Dt[2 * s * DP], buf[s], acc, 0, 0, 0);
-                if (2 * s + 2 < M4)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(Dt[(2 * s + 2) * DP], buf[s + 1], acc, 0, 0, 0);
+            for (int s0 = 0; s0 < 64; s0 += 16) {
+                if (2 * s0 < M4) {
+                    float d[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) d[u] = Dt[min(2 * (s0 + u), M4 - 2) * DP];   // rows past M4: unused
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        if (2 * (s0 + u) < M4) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], buf[s0 + u], acc, 0, 0, 0);
+                }
             }
             const bool ok = c < NCOL;
 #pragma unroll
