@@ -22,9 +22,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM traffic per launch of each kernel of this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs of this script (written by fftso3_amd.profiling)
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "so3_b64_batch256_fp32_pmc.json")
-# kernels of the batch-wide FP32 path at l_max = 63, by stage
-STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd<false>", "contract_fwd": "contract_fwd_mx32<4, true>",
-                "contract_inv": "contract_inv_mx32<4>", "fft_inv": "so3_fft128_inv<false>"}
+# kernels of the batch-wide FP32 path at l_max = 63 with its default schedule (group interleave 1:
+# paired-function FFTs, forward contraction over the two groups that share a workspace line), by stage
+STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd2<false>", "contract_fwd": "contract_fwd_mx32g<2>",
+                "contract_inv": "contract_inv_mx32<4>", "fft_inv": "so3_fft128_inv2<false>"}
 
 
 def log(msg):
@@ -297,7 +298,8 @@ def main():
     bytes_launch = alg_bytes(B, C, 8, dom)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
-    kname = STAGE_KERNEL[dom] if args.precision == "fp32" and args.chunk is None else dom
+    default_sched = args.precision == "fp32" and args.chunk is None and " gil=1 " in plan.describe()
+    kname = STAGE_KERNEL[dom] if default_sched else dom
     if kname in STAGE_KERNEL.values() and n_fn == 256 and os.path.exists(PMC_SUMMARY):
         k = json.load(open(PMC_SUMMARY))["kernels"].get(kname)
         traffic = k["traffic_bytes"] if k else None

@@ -3,13 +3,17 @@
 Usage (after three profiling runs of the same bench.py command; each summary records them):
 
     python -m fftso3_amd.profiling STATS_DIR FETCH_DIR WRITE_DIR OUT_PREFIX [--copy-bytes N]
+        [--mfma-dir DIR] [--stats-md FILE]
 
 STATS_DIR holds a `rocprofv3 --kernel-trace --stats --output-format csv` run, FETCH_DIR and
 WRITE_DIR `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs.  FETCH_SIZE and WRITE_SIZE
 are read in KiB per dispatch and scaled by the factor that makes the largest device copy of the
 run (bench.py --calibrate-copy: a copy of N bytes) read N bytes and write N bytes.  Writes
 OUT_PREFIX.md (a short table) and OUT_PREFIX.json ({kernel: {avg_ms, fetch_bytes, write_bytes,
-traffic_bytes}}).
+traffic_bytes}}).  With --mfma-dir (a `rocprofv3 --pmc SQ_INSTS_VALU_MFMA_MOPS_F32` run) each
+kernel also gets its FP32 MFMA work per launch (the counter counts units of 512 flops) and the
+rate that makes over the kernel's average duration.  --stats-md writes the --stats table (calls,
+average / min / max duration, share of GPU time) as markdown.
 """
 import argparse
 import collections
@@ -43,6 +47,24 @@ def counter_per_dispatch(d, counter):
     return {k: list(v.values()) for k, v in per.items()}
 
 
+def kernel_stats_full(d):
+    """Rows of a --stats run: (name, calls, avg ns, min ns, max ns, percent)."""
+    rows = []
+    for f in _csvs(d, "*kernel_stats.csv"):
+        for r in csv.DictReader(open(f)):
+            rows.append((r["Name"], int(r["Calls"]), float(r["AverageNs"]), float(r["MinNs"]), float(r["MaxNs"]),
+                         float(r["Percentage"])))
+    return sorted(rows, key=lambda x: -x[5])
+
+
+def stats_markdown(d, command):
+    lines = ["Command:", "", "```", command, "```", "",
+             "| kernel | calls | avg ms | min ms | max ms | % of GPU time |", "|---|---|---|---|---|---|"]
+    for n, c, a, lo, hi, pct in kernel_stats_full(d):
+        lines.append("| %s | %d | %.3f | %.3f | %.3f | %.2f |" % (short(n), c, a * 1e-6, lo * 1e-6, hi * 1e-6, pct))
+    return "\n".join(lines) + "\n"
+
+
 def short(name):
     return name.split("(")[0].replace("void ", "")
 
@@ -70,12 +92,28 @@ def summarize(stats_dir, fetch_dir, write_dir, copy_bytes):
     return {"fetch_scale": f_fetch, "write_scale": f_write, "copy_bytes": copy_bytes, "kernels": table}
 
 
+def add_mfma(s, mfma_dir):
+    """FP32 MFMA flops per launch (SQ_INSTS_VALU_MFMA_MOPS_F32 x 512) and their rate."""
+    mops = counter_per_dispatch(mfma_dir, "SQ_INSTS_VALU_MFMA_MOPS_F32")
+    for k, vs in mops.items():
+        v = s["kernels"].get(short(k))
+        if v is None:
+            continue
+        v["mfma_f32_flops"] = sum(vs) / len(vs) * 512.0
+        v["mfma_f32_tflops"] = v["mfma_f32_flops"] / (v["avg_ms"] * 1e-3) / 1e12
+
+
 def to_markdown(s, command):
-    lines = ["| kernel | calls | avg ms | HBM read (GB) | HBM write (GB) | traffic (GB) |",
-             "|---|---|---|---|---|---|"]
+    mf = any("mfma_f32_flops" in v for v in s["kernels"].values())
+    lines = ["| kernel | calls | avg ms | HBM read (GB) | HBM write (GB) | traffic (GB) |" +
+             (" FP32 MFMA GFLOP | MFMA TFLOP/s |" if mf else ""),
+             "|---|---|---|---|---|---|" + ("---|---|" if mf else "")]
     for k, v in sorted(s["kernels"].items(), key=lambda kv: -(kv[1]["avg_ms"] * max(kv[1]["calls"], 1))):
-        lines.append("| %s | %d | %.3f | %.3f | %.3f | %.3f |" % (
-            k, v["calls"], v["avg_ms"], v["fetch_bytes"] / 1e9, v["write_bytes"] / 1e9, v["traffic_bytes"] / 1e9))
+        row = "| %s | %d | %.3f | %.3f | %.3f | %.3f |" % (
+            k, v["calls"], v["avg_ms"], v["fetch_bytes"] / 1e9, v["write_bytes"] / 1e9, v["traffic_bytes"] / 1e9)
+        if mf:
+            row += " %.2f | %.1f |" % (v.get("mfma_f32_flops", 0.0) / 1e9, v.get("mfma_f32_tflops", 0.0))
+        lines.append(row)
     head = ["Command:", "", "```", command, "```", "",
             "FETCH_SIZE scaled by %.3f and WRITE_SIZE by %.3f (a %d-byte device copy read and written)." % (
                 s["fetch_scale"], s["write_scale"], s["copy_bytes"]), ""]
@@ -90,8 +128,15 @@ def main():
     ap.add_argument("out_prefix")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--command", default="")
+    ap.add_argument("--mfma-dir", default=None)
+    ap.add_argument("--stats-md", default=None)
     a = ap.parse_args()
     s = summarize(a.stats_dir, a.fetch_dir, a.write_dir, a.copy_bytes)
+    if a.mfma_dir:
+        add_mfma(s, a.mfma_dir)
+    if a.stats_md:
+        with open(a.stats_md, "w") as f:
+            f.write(stats_markdown(a.stats_dir, a.command))
     with open(a.out_prefix + ".json", "w") as f:
         json.dump(s, f, indent=1)
     with open(a.out_prefix + ".md", "w") as f:
