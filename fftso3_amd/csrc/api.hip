@@ -677,6 +677,17 @@ static bool fwd_grouped()
     return on != 0;
 }
 
+// inverse contraction over the two groups of each interleaved workspace line in one workgroup
+// (FFTSO3_INV_GROUPED=0 selects one group per workgroup)
+static bool inv_grouped()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_INV_GROUPED");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 template <int NP>
 int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *in, void *out, hipStream_t s)
 {
@@ -757,7 +768,17 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         if (c->mxk)
             hipLaunchKernelGGL((contract_inv_mxk<NP>), dim3(mgrid), dim3(256),
                                mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a);
-        else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
+        else if (c->N == 128 && NP == 4 && ca.gil == 1 && inv_grouped()) {
+            // the two groups that share workspace lines in one workgroup
+            MxArgs ag = a;
+            ag.gpw = 2;
+            ag.nitems = c->nitems_mx / 2;
+            const int nfbg = (ca.nfc + 7) / 8;
+            ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
+            const size_t glds = (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
+            hipLaunchKernelGGL((contract_inv_mx32g<2>), dim3(deal_grid(ag.nitems * ag.nsplit, ag.nsplit)), dim3(256),
+                               glds, s, ag);
+        } else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
         HIP_TRY(hipGetLastError());
     }
     return FFTSO3_OK;
@@ -1009,6 +1030,7 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32<4, true>));
     ALLOW_LDS((contract_fwd_mx32g<2>));
     ALLOW_LDS((contract_fwd_mx32g<4>));
+    ALLOW_LDS((contract_inv_mx32g<2>));
     ALLOW_LDS((contract_fwd_tab<4>));
     ALLOW_LDS((contract_fwd_tab<2>));
     ALLOW_LDS((contract_fwd_mx32<2, true>));

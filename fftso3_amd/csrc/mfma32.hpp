@@ -510,6 +510,124 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
     }
 }
 
+// Inverse, N = 128, SO(3), workspace with two groups interleaved (GPW = 2): one workgroup takes
+// the two groups whose records share each workspace line, so a block (8 functions of both groups)
+// is one contiguous 512-byte run of every beta row and each 128-byte line is written whole by one
+// store instruction.  Waves 0-1 work on the first group, 2-3 on the second, two pairs each; the
+// two groups' c_l d^l blocks sit side by side in LDS.
+template <int GPW>
+__global__ void __launch_bounds__(256) contract_inv_mx32g(MxArgs a)
+{
+    static_assert(GPW == 2, "the two groups of one interleaved workspace line");
+    constexpr int NP = 4, FB = 16 / GPW, WPG = 4 / GPW;  // functions per block, waves per group
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *D = reinterpret_cast<float *>(smem_raw);     // [GPW][64][MX_DPITCH], rows l - m0, columns k
+    float *O = D + GPW * 64 * MX_DPITCH;                 // [128][MX_LPITCH], output block
+    RecCoef *rcs = reinterpret_cast<RecCoef *>(O + 128 * MX_LPITCH);   // [GPW][B]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, C = ca.C, CW = C * NP * 2;
+    const int nfb = (ca.nfc + FB - 1) / FB;
+    int gb, fb0, fb1;
+    if (!mx_unit(a, nfb, gb, fb0, fb1)) return;         // gb: first (even) group of the set
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    const int gw = wave / WPG;                           // this wave's group within the set
+    int m0, n0;
+    so3_group(gb + gw, m0, n0);
+    const int M = B - m0, M4 = (M + 3) & ~3;
+    // operand role of this lane: pair p, function fo of the block, re/im part cl & 1
+    const int p = (wave % WPG) * GPW + (cl >> 4), fo = (cl & 15) >> 1;
+    const float *PT = reinterpret_cast<const float *>(a.src);
+    const MxPair<NP> pr(m0, n0, p);
+    int eoff[32];
+#pragma unroll
+    for (int s = 0; s < 32; s++) eoff[s] = pr.packed(m0 + min(2 * s + kh, M - 1));
+    float bA[32], bB[32];
+    auto load_block = [&](float *buf, int fb) {
+        const int fn = fb * FB + fo;
+        const float *src = PT + 2 * (fn < ca.nfc ? fn : 0) + (cl & 1);
+#pragma unroll
+        for (int s = 0; s < 32; s++) buf[s] = src[2L * eoff[s] * C];
+    };
+    load_block(bA, fb0);
+    {
+        // threads 0-127 recurse for the first group, 128-255 for the second: one beta row each
+        const int h = tid >> 7, k = tid & 127;
+        int hm0, hn0;
+        so3_group(gb + h, hm0, hn0);
+        RecCoef *rc = rcs + h * B;
+        float *Dh = D + h * 64 * MX_DPITCH;
+        group_coefs(a.rt, 0, B, hm0, hn0, rc, k, 128);
+        __syncthreads();
+        const int hM = B - hm0, hM4 = (hM + 3) & ~3;
+        group_recursion_lds(a.rt, rc, 0, gb + h, hm0, hn0, k, B, [&](int l, double v) {
+            Dh[(l - hm0) * MX_DPITCH + k] = (float)((double)(2 * l + 1) * v);
+        });
+        for (int r = hM; r < hM4; r++) Dh[r * MX_DPITCH + k] = 0.f;
+    }
+    const float sg = (float)so3_pair_sign(m0, n0, p);
+    // store role: rows r0 + 8i (i < 16) of the block, columns 4q..4q+3 = function q / 4, group
+    // (q / 2) & 1, pairs 2(q & 1) and 2(q & 1) + 1; computed by wave wq, lanes 2fq + {0,1} and 16 + 2fq + {0,1}
+    const int q = tid & 31, r0 = tid >> 5;
+    const int fq = q >> 2, wq = ((q >> 1) & 1) * WPG + (q & 1);
+    const int lc0 = wq * 32 + 2 * fq, lc1 = lc0 + 16;
+    float *G = reinterpret_cast<float *>(a.dst);
+    long roff[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, gb, r0 + 8 * i) * CW + 4 * q;
+    const float *Dg = D + gw * 64 * MX_DPITCH + kh * MX_DPITCH + cl;
+    auto multiply_store = [&](const float *buf, int fb) {
+        f32x16m_t acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int j = 0; j < 16; j++) acc[t][j] = 0.f;
+        // degrees l >= M read finite coefficients (row M-1) against zero rows of D up to M4
+#pragma unroll
+        for (int s = 0; s < 32; s += 2) {
+            if (4 * (s >> 1) < M4) {
+                const float *Dr = Dg + 2 * s * MX_DPITCH;
+                float d[2][4];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) d[u][t] = Dr[2 * u * MX_DPITCH + 32 * t];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u][t], buf[s + u], acc[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();                                 // previous block's rows stored
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int k = t * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
+                O[k * MX_LPITCH + wave * 32 + cl] = sg * acc[t][j];
+            }
+        __syncthreads();
+        const bool ok = fb * FB + fq < ca.nfc;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const float *row = O + (r0 + 8 * i) * MX_LPITCH;
+            const float2 u = *reinterpret_cast<const float2 *>(row + lc0);
+            const float2 v = *reinterpret_cast<const float2 *>(row + lc1);
+            float *d = ok ? G + roff[i] + (long)fb * 128 : a.dummy;
+            *reinterpret_cast<float4 *>(d) = make_float4(u.x, u.y, v.x, v.y);
+        }
+    };
+    __syncthreads();                                     // D written
+    for (int fb = fb0; fb < fb1; fb += 2) {
+        if (fb + 1 < fb1) load_block(bB, fb + 1);
+        multiply_store(bA, fb);
+        if (fb + 1 >= fb1) break;
+        if (fb + 2 < fb1) load_block(bA, fb + 2);
+        multiply_store(bB, fb + 1);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Coefficient transpose between the reference's packed layout (FFTSO3_matrix / FFTS2_matrix
 // storage, one function after another: [function][e]) and PT[e][function].  Tiles of 64 packed
