@@ -677,6 +677,16 @@ static bool fwd_grouped()
     return on != 0;
 }
 
+// grouped forward contraction on 16 x 16 x 4 MFMA tiles (FFTSO3_FWD_MX16=0: 32 x 32 x 2 tiles)
+static bool fwd_mx16()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FWD_MX16");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 // inverse contraction over the two groups of each interleaved workspace line in one workgroup
 // (FFTSO3_INV_GROUPED=0 selects one group per workgroup)
 static bool inv_grouped()
@@ -735,7 +745,8 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
                 ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
                 const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
                 const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
-                if (ag.gpw == 2) hipLaunchKernelGGL((contract_fwd_mx32g<2>), dim3(ggrid), dim3(256), glds, s, ag);
+                if (ag.gpw == 2 && fwd_mx16()) hipLaunchKernelGGL(contract_fwd_mx16g, dim3(ggrid), dim3(256), glds, s, ag);
+                else if (ag.gpw == 2) hipLaunchKernelGGL((contract_fwd_mx32g<2>), dim3(ggrid), dim3(256), glds, s, ag);
                 else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
             } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
             else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
@@ -1031,6 +1042,7 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32g<2>));
     ALLOW_LDS((contract_fwd_mx32g<4>));
     ALLOW_LDS((contract_inv_mx32g<2>));
+    ALLOW_LDS(contract_fwd_mx16g);
     ALLOW_LDS((contract_fwd_tab<4>));
     ALLOW_LDS((contract_fwd_tab<2>));
     ALLOW_LDS((contract_fwd_mx32<2, true>));

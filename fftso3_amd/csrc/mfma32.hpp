@@ -19,6 +19,8 @@
 // D[8(j/4) + 4(i/32) + j%4][i%32] for j < 16.
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 typedef float f32x16m_t __attribute__((ext_vector_type(16)));
@@ -376,6 +378,139 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32g(MxArgs a)
             float *d1 = ok && r + 32 < M ? o + 2 * e1 * C : a.dummy;
             *d0 = sg * acc0[j];
             *d1 = sg * acc1[j];
+        }
+    }
+}
+
+typedef float f32x4m_t __attribute__((ext_vector_type(4)));
+
+// Forward as contract_fwd_mx32g<2>, on 16 x 16 x 4 MFMA tiles: a group's degree rows are padded to
+// a multiple of 16 instead of 32, which halves the MFMA work of the groups with at most 16 degrees
+// (44% of the groups at l_max = 63).  Tile layouts (lane i): A[i%16][i/16] = D(l, k), B[i/16][i%16]
+// = spectrum(k, column), result row 4(i/16) + j, column i%16.  Wave w takes group w / 2 and pairs
+// 2(w % 2), 2(w % 2) + 1: one 16-column tile per pair (8 functions, re/im).
+__global__ void __launch_bounds__(256) contract_fwd_mx16g(MxArgs a)
+{
+    constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *L = reinterpret_cast<float *>(smem_raw);     // [128][MX_LPITCH], spectrum block
+    float *Dst = L;                                      // [2][64][MX_DPITCH] staging, before L is used
+    RecCoef *rcs = reinterpret_cast<RecCoef *>(smem_raw + 128 * MX_LPITCH * 4);   // [2][B]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
+    const int nfb = (ca.nfc + FB - 1) / FB;
+    int gb, fb0, fb1;
+    if (!mx_unit(a, nfb, gb, fb0, fb1)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kq = lane >> 4, cx = lane & 15;
+    const int gg = wave / WPG, pg = wave % WPG;
+    int m0, n0;
+    so3_group(gb + gg, m0, n0);
+    const int M = B - m0, nrt = (M + 15) / 16;
+    // load role (as contract_fwd_mx32g<2>): rows r0 + 8i of the block, columns 4q..4q+3
+    const int q = tid & 31, r0 = tid >> 5;
+    const int fl_q = (q >> 1) / GPW, gq = (q >> 1) % GPW;
+    auto lcol = [](int fl, int gq_, int p) { return (gq_ * WPG + p / GPW) * 32 + (p % GPW) * FB * 2 + fl * 2; };
+    const int lc0 = lcol(fl_q, gq, 2 * (q & 1)), lc1 = lcol(fl_q, gq, 2 * (q & 1) + 1);
+    const float *S = reinterpret_cast<const float *>(a.src);
+    long roff[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, gb, min(r0 + 8 * i, N - 1)) * CW + 4 * q;
+    float4 pf[16];
+    auto load_block = [&](int fb) {
+        const long co = fb * FB + fl_q < ca.nfc ? (long)fb * 128 : 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + roff[i] + co);
+    };
+    load_block(fb0);
+    // D of the two groups: threads 0-127 recurse for the first, 128-255 the second
+    float dR[4][32];
+    {
+        const int h = tid >> 7;
+        int sm0, sn0;
+        so3_group(gb + h, sm0, sn0);
+        RecCoef *rc = rcs + h * B;
+        float *Dh = Dst + h * 64 * MX_DPITCH;
+        group_coefs(a.rt, 0, B, sm0, sn0, rc, tid & 127, 128);
+        __syncthreads();
+        const int k = tid & 127, sM = B - sm0;
+        const double wk = a.rt.w[k];
+        group_recursion_lds(a.rt, rc, 0, gb + h, sm0, sn0, k, B, [&](int l, double v) {
+            Dh[(l - sm0) * MX_DPITCH + k] = (float)(wk * v);
+        });
+        for (int r = sM; r < 16 * ((sM + 15) / 16); r++) Dh[r * MX_DPITCH + k] = 0.f;
+        __syncthreads();
+        const float *D0 = Dst + gg * 64 * MX_DPITCH + cx * MX_DPITCH + kq;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int s = 0; s < 32; s++) dR[t][s] = t < nrt ? D0[16 * t * MX_DPITCH + 4 * s] : 0.f;
+        __syncthreads();                                 // staging read before it is overwritten
+    }
+    // output role of this lane: pairs pg*2 + {0, 1} (column tiles 0, 1), function fo, re/im cx & 1
+    const int fo = cx >> 1;
+    const MxPair<NP> pr0(m0, n0, pg * 2), pr1(m0, n0, pg * 2 + 1);
+    const float sg0 = (float)so3_pair_sign(m0, n0, pg * 2), sg1 = (float)so3_pair_sign(m0, n0, pg * 2 + 1);
+    const bool pv0 = so3_pair_valid(m0, n0, pg * 2), pv1 = so3_pair_valid(m0, n0, pg * 2 + 1);
+    float *PT = reinterpret_cast<float *>(a.dst);
+    const float *Lw = L + kq * MX_LPITCH + wave * 32 + cx;
+    for (int fb = fb0; fb < fb1; fb++) {
+        __syncthreads();                                 // previous block consumed
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            float *row = L + (r0 + 8 * i) * MX_LPITCH;
+            *reinterpret_cast<float2 *>(row + lc0) = make_float2(pf[i].x, pf[i].y);
+            *reinterpret_cast<float2 *>(row + lc1) = make_float2(pf[i].z, pf[i].w);
+        }
+        __syncthreads();
+        if (fb + 1 < fb1) load_block(fb + 1);
+        f32x4m_t acc[4][2];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[t][c][j] = 0.f;
+        auto run = [&](auto R) {
+            constexpr int NR = decltype(R)::value;
+#pragma unroll
+            for (int s0 = 0; s0 < 32; s0 += 4) {
+                float b[4][2];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    b[u][0] = Lw[4 * (s0 + u) * MX_LPITCH];
+                    b[u][1] = Lw[4 * (s0 + u) * MX_LPITCH + 16];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int t = 0; t < NR; t++) {
+                        acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(dR[t][s0 + u], b[u][0], acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(dR[t][s0 + u], b[u][1], acc[t][1], 0, 0, 0);
+                    }
+            }
+        };
+        if (nrt == 1) run(std::integral_constant<int, 1>());
+        else if (nrt == 2) run(std::integral_constant<int, 2>());
+        else if (nrt == 3) run(std::integral_constant<int, 3>());
+        else run(std::integral_constant<int, 4>());
+        const int fn = fb * FB + fo;
+        const bool okf = fn < ca.nfc;
+        float *o = PT + 2 * fn + (cx & 1);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (t < nrt) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int r = 16 * t + 4 * kq + j;
+                    const long e0 = pr0.packed(m0 + min(r, M - 1)), e1 = pr1.packed(m0 + min(r, M - 1));
+                    float *d0 = okf && pv0 && r < M ? o + 2 * e0 * C : a.dummy;
+                    float *d1 = okf && pv1 && r < M ? o + 2 * e1 * C : a.dummy;
+                    *d0 = sg0 * acc[t][0][j];
+                    *d1 = sg1 * acc[t][1][j];
+                }
+            }
         }
     }
 }
