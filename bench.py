@@ -23,9 +23,9 @@ HBM_PEAK_GBS = 8000.0
 # runs of this script (written by fftso3_amd.profiling)
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "so3_b64_batch256_fp32_pmc.json")
 # kernels of the batch-wide FP32 path at l_max = 63 with its default schedule (group interleave 1:
-# paired-function FFTs, forward contraction over the two groups that share a workspace line), by stage
-STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd2<false>", "contract_fwd": "contract_fwd_mx32g<2>",
-                "contract_inv": "contract_inv_mx32<4>", "fft_inv": "so3_fft128_inv2<false>"}
+# paired-function FFTs, contractions over the two groups that share a workspace line), by stage
+STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd2<false>", "contract_fwd": "contract_fwd_mx16g",
+                "contract_inv": "contract_inv_mx32g<2>", "fft_inv": "so3_fft128_inv2<false>"}
 
 
 def log(msg):
