@@ -1199,7 +1199,8 @@ __global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
             if (mx_pair_valid<NP>(m0, n0, p)) {
                 const long idx = MxPair<NP>(m0, n0, p).packed(m0 + lr);
                 const float2 f = a.pt ? Fc[idx * C + fi] : Fc[ca.coef_off[ca.fn[fi]] + idx];
-                v = (float)mx_pair_sign<NP>(m0, n0, p) * ((c & 1) ? f.y : f.x);
+                const float cf = NP == 2 ? 1.f : (float)(2 * (m0 + lr) + 1);   // SO(3): (2l+1)
+                v = (float)mx_pair_sign<NP>(m0, n0, p) * cf * ((c & 1) ? f.y : f.x);
             }
         }
         Cb[lr * TAB_CPITCH + c] = v;
@@ -1212,22 +1213,34 @@ __global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
     f32x16m_t acc0, acc1;
 #pragma unroll
     for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
-    for (int s0 = 0; 2 * s0 < M4; s0 += 8) {
-        float av[8];
+    // 16 steps per batch, the next batch's table loads in flight during this batch's MFMAs.
+    // Degree rows past M read row M - 1 (the address is clamped, not the value) against the zero
+    // rows of the coefficient block.
+    constexpr int SB = 16;
+    const int nst = M4 / 2;
+    const float *trow = tab + kc + (long)kh * N;
+    auto load_batch = [&](float *av, int s0) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int lr = 2 * (s0 + u) + kh;
-            const float cf = NP == 2 ? 1.f : (float)(2 * (m0 + lr) + 1);
-            av[u] = lr < M ? cf * tab[(long)lr * N + kc] : 0.f;
-        }
+        for (int u = 0; u < SB; u++) av[u] = trow[(long)min(2 * (s0 + u), M - 1 - kh) * N];
+    };
+    auto mult_batch = [&](const float *av, int s0) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            if (2 * (s0 + u) < M4) {
+        for (int u = 0; u < SB; u++) {
+            if (s0 + u < nst) {
                 const float *cb = Cb + (2 * (s0 + u) + kh) * TAB_CPITCH + cl;
                 acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], cb[0], acc0, 0, 0, 0);
                 if (nct > 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av[u], cb[32], acc1, 0, 0, 0);
             }
         }
+    };
+    float aA[SB], aB[SB];
+    load_batch(aA, 0);
+    for (int s0 = 0; s0 < nst; s0 += 2 * SB) {
+        if (s0 + SB < nst) load_batch(aB, s0 + SB);
+        mult_batch(aA, s0);
+        if (s0 + SB >= nst) break;
+        if (s0 + 2 * SB < nst) load_batch(aA, s0 + 2 * SB);
+        mult_batch(aB, s0 + SB);
     }
     float *G = reinterpret_cast<float *>(a.dst);
 #pragma unroll
