@@ -584,6 +584,16 @@ FftArgs<T> fft_args(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, const void
 
 
 // table-fed MFMA forward contraction above l_max = 127 (FFTSO3_FWD_TAB=0 selects the vector kernel)
+// half-staged table forward (FFTSO3_FWD_TAB2=0: the whole spectrum block in LDS at once)
+static bool fwd_tab2()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FWD_TAB2");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 static bool fwd_tab()
 {
     static const int on = [] {
@@ -613,9 +623,16 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
         ConArgs<float> af;
         std::memcpy(&af, &a, sizeof(af));
         const int ngrp = a.s2 ? c->B : c->B * c->B;
-        const size_t lds = (size_t)c->N * TABF_COLS * sizeof(float);
-        if (a.s2) hipLaunchKernelGGL((contract_fwd_tab<2>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
-        else hipLaunchKernelGGL((contract_fwd_tab<4>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+        if (c->B <= 32 * 4 * TABF2_TPW && c->B % 8 == 0 && fwd_tab2()) {
+            // the spectrum block staged half a beta range at a time: two workgroups per CU
+            const size_t lds = (size_t)(c->N / 2) * TABF_COLS * sizeof(float);
+            if (a.s2) hipLaunchKernelGGL((contract_fwd_tab2<2>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+            else hipLaunchKernelGGL((contract_fwd_tab2<4>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+        } else {
+            const size_t lds = (size_t)c->N * TABF_COLS * sizeof(float);
+            if (a.s2) hipLaunchKernelGGL((contract_fwd_tab<2>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+            else hipLaunchKernelGGL((contract_fwd_tab<4>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
+        }
     } else if (fwd) {
         a.items = (const int2 *)c->d_items_fwd.p;
         a.nitems = c->nitems_fwd;
@@ -627,13 +644,24 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
         // FP32 with the d table: MFMA over 128-beta tiles
         ConArgs<float> af;
         std::memcpy(&af, &a, sizeof(af));
-        af.items = (const int2 *)c->d_items_inv_mx.p;
-        af.nitems = c->nitems_inv_mx;
         af.tile = 128;
-        const size_t lds = (size_t)((c->B + 3) & ~3) * TAB_CPITCH * sizeof(float);
-        const int tgrid = deal_grid(af.nitems, (c->N + 127) / 128);
-        if (a.s2) hipLaunchKernelGGL((contract_inv_tab<2>), dim3(tgrid), dim3(256), lds, s, af);
-        else hipLaunchKernelGGL((contract_inv_tab<4>), dim3(tgrid), dim3(256), lds, s, af);
+        // Items run largest group first.  The coefficient block takes M4 x TAB_CPITCH floats of
+        // LDS, so the groups with at most 128 degrees (75% of them at l_max = 255) go in a launch
+        // of their own whose smaller LDS lets 4 workgroups share a CU instead of 2.
+        const int nk = (c->N + 127) / 128, MT = 128;
+        const int gsplit = c->B > MT ? (a.s2 ? c->B - MT : (c->B - MT) * (c->B - MT)) : (a.s2 ? c->B : c->B * c->B);
+        const int isplit = gsplit * nk;
+        for (int part = 0; part < 2; part++) {
+            const int i0 = part ? isplit : 0, i1 = part ? c->nitems_inv_mx : isplit;
+            if (i1 <= i0) continue;
+            af.items = (const int2 *)c->d_items_inv_mx.p + i0;
+            af.nitems = i1 - i0;
+            const int mmax = part ? MT : c->B;
+            const size_t lds = (size_t)((mmax + 3) & ~3) * TAB_CPITCH * sizeof(float);
+            const int tgrid = deal_grid(af.nitems, nk);
+            if (a.s2) hipLaunchKernelGGL((contract_inv_tab<2>), dim3(tgrid), dim3(256), lds, s, af);
+            else hipLaunchKernelGGL((contract_inv_tab<4>), dim3(tgrid), dim3(256), lds, s, af);
+        }
     } else {
         a.items = (const int2 *)c->d_items_inv.p;
         a.nitems = c->nitems_inv;
@@ -1045,6 +1073,8 @@ int set_kernel_attrs()
     ALLOW_LDS(contract_fwd_mx16g);
     ALLOW_LDS((contract_fwd_tab<4>));
     ALLOW_LDS((contract_fwd_tab<2>));
+    ALLOW_LDS((contract_fwd_tab2<4>));
+    ALLOW_LDS((contract_fwd_tab2<2>));
     ALLOW_LDS((contract_fwd_mx32<2, true>));
     ALLOW_LDS((contract_fwd_mx32<4, false>));
     ALLOW_LDS((contract_fwd_mx32<2, false>));

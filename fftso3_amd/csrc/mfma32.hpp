@@ -1171,6 +1171,121 @@ __global__ void __launch_bounds__(256) contract_fwd_tab(ConArgs<float> a)
     }
 }
 
+// Forward on the d table with the spectrum block staged one half of the beta range at a time:
+// 64 KB of LDS at l_max = 255, so two workgroups share a CU and one's staging overlaps the
+// other's MFMAs.  Within a half, step u of block j covers beta rows 8j + 4kh + u (kh = lane / 32),
+// so a lane reads its table row as one float4 per block; rows with bit 2 set are stored 32
+// columns rotated so the two lane halves' operand reads fall in opposite banks.  A wave keeps the
+// accumulators of up to two 32-degree tiles (t = wave, wave + 4) across both halves: B <= 256,
+// and B % 8 == 0 so that every block is whole.
+#define TABF2_TPW 2
+template <int NP>
+__global__ void __launch_bounds__(256) contract_fwd_tab2(ConArgs<float> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *Sl = reinterpret_cast<float *>(smem_raw);    // [H][TABF_COLS]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2, H = N / 2, nb = H / 8;
+    const int g = xcd_deal(1);
+    if (g >= ca.ngrp) return;
+    int m0, n0;
+    mx_group<NP>(g, m0, n0);
+    const int M = B - m0, nt = (M + 31) / 32;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    const float *tab = reinterpret_cast<const float *>(a.dtab) + a.dtab_off[g];
+    const int nct = NCOL > 32 ? 2 : 1;
+    const int c0 = cl, c1 = 32 + cl;
+    const float *Sb0 = Sl + 4 * kh * TABF_COLS + ((c0 + 32 * kh) & (TABF_COLS - 1));
+    const float *Sb1 = Sl + 4 * kh * TABF_COLS + ((c1 + 32 * kh) & (TABF_COLS - 1));
+    f32x16m_t acc[TABF2_TPW][2];
+#pragma unroll
+    for (int i = 0; i < TABF2_TPW; i++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) { acc[i][0][j] = 0.f; acc[i][1][j] = 0.f; }
+    const float4 *S4 = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(a.src) + ws_row(ca, g, 0) * CW);
+    const int q4 = CW / 4, ne = H * q4;
+    for (int h = 0; h < 2; h++) {
+        const int k0 = h * H;
+        __syncthreads();                                 // the previous half's operand reads are done
+        // stage w_k S_k of rows k0 .. k0 + H - 1, 8 loads in flight per thread
+        for (int e0 = 0; e0 < ne; e0 += 256 * 8) {
+            float4 v[8];
+            double w[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int e = min(e0 + tid + 256 * i, ne - 1);
+                v[i] = S4[(long)k0 * q4 + e];
+                w[i] = a.rt.w[k0 + e / q4];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int e = e0 + tid + 256 * i;
+                if (e < ne) {
+                    const int kk = e / q4, c = 4 * (e - kk * q4);
+                    const float4 u = make_float4((float)(w[i] * v[i].x), (float)(w[i] * v[i].y), (float)(w[i] * v[i].z),
+                                                 (float)(w[i] * v[i].w));
+                    *reinterpret_cast<float4 *>(Sl + kk * TABF_COLS + ((c + ((kk >> 2) & 1) * 32) & (TABF_COLS - 1))) = u;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TABF2_TPW; i++) {
+            const int t = wave + 4 * i;
+            if (t >= nt) break;
+            const int lr = 32 * t + cl;                   // this lane's A row (degree m0 + lr)
+            const float4 *t4 = reinterpret_cast<const float4 *>(tab + (long)min(lr, M - 1) * N + k0) + kh;
+            // rounds of 8 blocks (64 beta rows), the next round's 8 loads in flight
+            float4 cur[8], nxt[8];
+#pragma unroll
+            for (int b = 0; b < 8; b++) cur[b] = t4[2 * min(b, nb - 1)];
+            for (int j0 = 0; j0 < nb; j0 += 8) {
+                if (j0 + 8 < nb) {
+#pragma unroll
+                    for (int b = 0; b < 8; b++) nxt[b] = t4[2 * min(j0 + 8 + b, nb - 1)];
+                }
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    if (j0 + b < nb) {
+                        const float q[4] = {cur[b].x, cur[b].y, cur[b].z, cur[b].w};
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int row = (8 * (j0 + b) + u) * TABF_COLS;
+                            acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(q[u], Sb0[row], acc[i][0], 0, 0, 0);
+                            if (nct > 1) acc[i][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(q[u], Sb1[row], acc[i][1], 0, 0, 0);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 8; b++) cur[b] = nxt[b];
+            }
+        }
+    }
+    float *dst = reinterpret_cast<float *>(a.dst);
+    const long es = a.pt ? 2L * C : 2L;
+#pragma unroll
+    for (int i = 0; i < TABF2_TPW; i++) {
+        const int t = wave + 4 * i;
+        if (t >= nt) break;
+#pragma unroll
+        for (int hc = 0; hc < 2; hc++) {
+            const int c = hc ? c1 : c0;
+            if (c >= NCOL || (hc && nct < 2)) continue;
+            const int ri = c & 1, p = (c >> 1) % NP, fi = c / (2 * NP);
+            if (!mx_pair_valid<NP>(m0, n0, p)) continue;
+            const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
+            const MxPair<NP> pr(m0, n0, p);
+            float *out = dst + (a.pt ? 2L * fi : 2L * ca.coef_off[ca.fn[fi]]) + ri;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int r = 32 * t + 8 * (j >> 2) + 4 * kh + (j & 3);
+                if (r < M) out[es * pr.packed(m0 + r)] = sg * acc[i][hc][j];
+            }
+        }
+    }
+}
+
 #define TAB_CPITCH 65
 
 template <int NP>
