@@ -232,10 +232,11 @@ def test_batchwide_large_n_fp32(fftso3, torch_cuda, oracle, domain, B):
             assert rel_l2(c[i], F[i]) < FP32_TOL
 
 
-@pytest.mark.parametrize("domain,B", [("s2", 150), ("so3", 136)])
+@pytest.mark.parametrize("domain,B", [("s2", 150), ("s2", 152), ("so3", 136)])
 def test_table_path_fp32(fftso3, torch_cuda, oracle, domain, B):
-    """l_max above 127 on the chunked kernels with the plan's Wigner-d table (MFMA inverse),
-    complex and real, against the oracle."""
+    """l_max above 127 on the chunked kernels with the plan's Wigner-d table (MFMA inverse in two
+    launches split at 128 degrees; MFMA forward, half-staged when B is a multiple of 8: S^2 152,
+    SO(3) 136), complex and real, against the oracle."""
     if domain == "s2":
         F = inputs.s2_coefficients(B, 3, 1600 + B, scale=lambda l: 1.0 / (1.0 + l))
         g, c = s2_roundtrip(fftso3, torch_cuda, B, F, prec="fp32")
