@@ -715,6 +715,16 @@ static bool fwd_mx16()
     return on != 0;
 }
 
+// inverse mxk contraction with 8 waves per workgroup (FFTSO3_MXK_INV8=0: 4 waves)
+static bool mxk_inv8()
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_MXK_INV8");
+        return e ? atoi(e) : 1;
+    }();
+    return on != 0;
+}
+
 // inverse contraction over the two groups of each interleaved workspace line in one workgroup
 // (FFTSO3_INV_GROUPED=0 selects one group per workgroup)
 static bool inv_grouped()
@@ -805,8 +815,18 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         a.src = pl->d_fc.p;
         a.dst = pl->d_ws.p;
         if (c->mxk)
-            hipLaunchKernelGGL((contract_inv_mxk<NP>), dim3(mgrid), dim3(256),
-                               mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a);
+        {
+            if (mxk_inv8()) {
+                // 8 waves per workgroup over blocks of 256 columns: half as many units
+                MxArgs a8 = a;
+                const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+                a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
+                hipLaunchKernelGGL((contract_inv_mxk<NP, 8>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
+                                   dim3(512), mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a8);
+            } else
+                hipLaunchKernelGGL((contract_inv_mxk<NP, 4>), dim3(mgrid), dim3(256),
+                                   mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a);
+        }
         else if (c->N == 128 && NP == 4 && ca.gil == 1 && inv_grouped()) {
             // the two groups that share workspace lines in one workgroup
             MxArgs ag = a;
@@ -1083,8 +1103,10 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv_tab<2>));
     ALLOW_LDS((contract_fwd_mxk<4>));
     ALLOW_LDS((contract_fwd_mxk<2>));
-    ALLOW_LDS((contract_inv_mxk<4>));
-    ALLOW_LDS((contract_inv_mxk<2>));
+    ALLOW_LDS((contract_inv_mxk<4, 4>));
+    ALLOW_LDS((contract_inv_mxk<2, 4>));
+    ALLOW_LDS((contract_inv_mxk<4, 8>));
+    ALLOW_LDS((contract_inv_mxk<2, 8>));
     ALLOW_LDS((contract_inv_mx32<2>));
     ALLOW_LDS((so3_fft128_fwd<true>));
     ALLOW_LDS((so3_fft128_fwd<false>));

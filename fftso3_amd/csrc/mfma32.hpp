@@ -965,13 +965,15 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
 
 // Inverse with the group's c_l d^l block resident in LDS ([M4][N+1], rows l - m0 zero-padded to a
 // multiple of 4); wave w produces block columns [32w, 32w+32) for every beta row.
-template <int NP>
-__global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
+// NW waves per workgroup, each owning 32 columns of a block: 8 waves (2 per SIMD) share the
+// group's d block so one wave's MFMAs cover another's LDS and HBM waits.
+template <int NP, int NW>
+__global__ void __launch_bounds__(64 * NW) contract_inv_mxk(MxArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
-    const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
+    const int nfb = (NCOL + 32 * NW - 1) / (32 * NW);       // column blocks of 32 NW
     int g, fb0, fb1;
     if (!mx_unit(a, nfb, g, fb0, fb1)) return;
     int m0, n0;
@@ -981,14 +983,14 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
     int *etab = reinterpret_cast<int *>(D + (size_t)M4 * DP);       // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
-    for (int i = tid; i < NP * MXK_MAXROWS; i += 256) {
+    for (int i = tid; i < NP * MXK_MAXROWS; i += 64 * NW) {
         const int p = i / MXK_MAXROWS, r = i - p * MXK_MAXROWS;
         etab[i] = MxPair<NP>(m0, n0, p).packed(m0 + min(r, M - 1));
     }
     RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, B, NP, false));
-    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 256);
+    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 64 * NW);
     __syncthreads();
-    for (int k = tid; k < 32 * nkt; k += 256) {
+    for (int k = tid; k < 32 * nkt; k += 64 * NW) {
         if (k < N) {
             group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * DP + k] = (float)(NP == 2 ? v : (double)(2 * l + 1) * v);
@@ -1005,7 +1007,7 @@ __global__ void __launch_bounds__(256) contract_inv_mxk(MxArgs a)
     float *G = reinterpret_cast<float *>(a.dst);
     const int *et = etab + p * MXK_MAXROWS;
     for (int fb = fb0; fb < fb1; fb++) {
-        const int c = fb * 128 + cb, cv = c < NCOL ? c : 0;
+        const int c = fb * 32 * NW + cb, cv = c < NCOL ? c : 0;
         const float *src = PT + 2 * (cv / (2 * NP)) + (cv & 1);
         // coefficient rows l = m0 + 2s + kh (clamped to finite rows; zero rows of D beyond M)
         float buf[64];
