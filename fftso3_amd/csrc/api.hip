@@ -85,7 +85,8 @@ struct Class {
     DevBuf d_items_fwd_mx, d_items_inv_mx;   // MFMA kernels: 64-row and 128-k tiles
     int nitems_fwd_mx = 0, nitems_inv_mx = 0;
     bool fused = false;             // whole slice fits in LDS
-    int row_tile = 0, col_tile = 0; // two-pass fallback tiles
+    int row_tile = 0, col_tile = 0; // two-pass fallback tiles (S^2 rings: divide N)
+    int so3_tile = 0;               // SO(3) row / column passes (partial last block)
     bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
     bool mxk = false;               // ... with the K-chunked kernels (128 < N, B <= 128)
     bool pt = false;                // chunked contraction reads / writes transposed coefficients PT
@@ -393,6 +394,11 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     while (rt * 2 <= N && N % (rt * 2) == 0 && (size_t)(rt * 2) * N <= cap) rt *= 2;
     c->row_tile = rt;
     c->col_tile = rt;
+    // SO(3) row / column passes: the largest power-of-two tile that fits, whether or not it
+    // divides N (the last block is partial): N = 200 gets 64 rows instead of 8
+    int rt3 = 1;
+    while (rt3 * 2 <= N && (size_t)(rt3 * 2) * N <= cap) rt3 *= 2;
+    c->so3_tile = rt3;
     return FFTSO3_OK;
 }
 // log2 of the run of beta rows a group keeps contiguous in the batch-wide workspace
@@ -919,14 +925,14 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                 if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
             } else {
-                a.tile = c->row_tile;
-                int nunit = nfc * N * (N / a.tile);
+                a.tile = c->so3_tile;
+                int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
                 size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((so3_fft_rows<T, false, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 HIP_TRY(hipGetLastError());
-                a.tile = c->col_tile;
-                nunit = nfc * N * (N / a.tile);
+                a.tile = c->so3_tile;
+                nunit = nfc * N * ((N + a.tile - 1) / a.tile);
                 lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 hipLaunchKernelGGL((so3_fft_cols<T, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
             }
@@ -1008,13 +1014,13 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                 if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
             } else {
-                a.tile = c->col_tile;
-                int nunit = nfc * N * (N / a.tile);
+                a.tile = c->so3_tile;
+                int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
                 size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 hipLaunchKernelGGL((so3_fft_cols<T, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 HIP_TRY(hipGetLastError());
-                a.tile = c->row_tile;
-                nunit = nfc * N * (N / a.tile);
+                a.tile = c->so3_tile;
+                nunit = nfc * N * ((N + a.tile - 1) / a.tile);
                 lds = (size_t)a.tile * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
                 else hipLaunchKernelGGL((so3_fft_rows<T, false, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);

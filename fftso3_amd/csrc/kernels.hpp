@@ -229,13 +229,16 @@ __global__ void __launch_bounds__(1024) so3_fft_rows(FftArgs<T> a)
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
-    const int N = ca.N, R = a.tile, nblk = N / R, nunit = ca.nfc * N * nblk;
+    const int N = ca.N, R = a.tile, nblk = (N + R - 1) / R, nunit = ca.nfc * N * nblk;
     T2 *buf = reinterpret_cast<T2 *>(smem_raw);
     T2 *twl = buf + R * N;
     int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
     if (unit >= nunit) return;
     const int rb = unit % nblk, k = (unit / nblk) % N, fi = unit / (nblk * N);
+    // rows of this block: the last block may be partial; the rows past it are transformed along
+    // with the rest (each row's DFT is independent) and never stored
+    const int nrow = min(R, N - rb * R);
     const int fn = ca.fn[fi];
     const int tid = threadIdx.x, nthr = blockDim.x;
     load_fft_tables(twl, revl, a.tw, a.rev, N, tid, nthr);
@@ -244,18 +247,18 @@ __global__ void __launch_bounds__(1024) so3_fft_rows(FftArgs<T> a)
     if (MODE == 0) {
         if (REAL) {
             const T *g = reinterpret_cast<const T *>(a.src) + gbase;
-            fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return cmk<T2>(g[e], (T)0); });
+            fill_lds<8>(buf, nrow * N, tid, nthr, [&](int e) { return cmk<T2>(g[e], (T)0); });
         } else {
             const T2 *g = reinterpret_cast<const T2 *>(a.src) + gbase;
-            fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return g[e]; });
+            fill_lds<8>(buf, nrow * N, tid, nthr, [&](int e) { return g[e]; });
         }
     } else {
         const T2 *sc = a.scratch + sbase;
-        fill_lds<8>(buf, R * N, tid, nthr, [&](int e) { return sc[e]; });
+        fill_lds<8>(buf, nrow * N, tid, nthr, [&](int e) { return sc[e]; });
     }
     __syncthreads();
     fft_tile(buf, a.fd, 1, N, R, true, twl, tid, nthr);
-    for (int e = tid; e < R * N; e += nthr) {
+    for (int e = tid; e < nrow * N; e += nthr) {
         int r = e / N, v = e - r * N;
         T2 val = buf[r * N + revl[v]];
         if (MODE == 0) a.scratch[sbase + e] = val;
@@ -273,8 +276,8 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
     typedef typename cplx_t<T>::type T2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
-    const int N = ca.N, B = ca.B, TC = a.tile, nblk = N / TC, nunit = ca.nfc * N * nblk;
-    T2 *buf = reinterpret_cast<T2 *>(smem_raw);  // [row][tc]
+    const int N = ca.N, B = ca.B, TC = a.tile, nblk = (N + TC - 1) / TC, nunit = ca.nfc * N * nblk;
+    T2 *buf = reinterpret_cast<T2 *>(smem_raw);  // [row][tc]; columns past N (last block) stay zero
     T2 *twl = buf + TC * N;
     int *revl = reinterpret_cast<int *>(twl + N);
     const int unit = xcd_unit(nunit);
@@ -295,14 +298,14 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
     if (MODE == 0) {
         const T2 *sc = a.scratch + sbase;
         fill_lds<8>(buf, N * TC, tid, nthr, [&](int e) {
-            int r = e / TC, c = e - r * TC;
-            return sc[(long)r * N + cb * TC + c];
+            int r = e / TC, c = e - r * TC, v = cb * TC + c;
+            return v < N ? sc[(long)r * N + v] : cmk<T2>((T)0, (T)0);
         });
     } else {
         fill_lds<8>(buf, N * TC, tid, nthr, [&](int e) {
             int u = e / TC, c = e - u * TC, v = cb * TC + c;
             T2 val = cmk<T2>((T)0, (T)0);
-            if (u != B && v != B) {
+            if (u != B && v != B && v < N) {
                 int m = u < B ? u : u - N, n = v < B ? v : v - N, g, p;
                 so3_slot(m, n, g, p);
                 val = W[ws_at(ca, g, k, p, fi, 4)];
@@ -315,7 +318,7 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
     if (MODE == 0) {
         for (int e = tid; e < N * TC; e += nthr) {
             int u = e / TC, c = e - u * TC, v = cb * TC + c;
-            if (u == B || v == B) continue;
+            if (u == B || v == B || v >= N) continue;
             int m = wrapN(-u, N), n = wrapN(-v, N);
             m = m < B ? m : m - N;
             n = n < B ? n : n - N;
@@ -328,7 +331,7 @@ __global__ void __launch_bounds__(1024) so3_fft_cols(FftArgs<T> a)
     } else {
         for (int e = tid; e < N * TC; e += nthr) {
             int j1 = e / TC, c = e - j1 * TC;
-            a.scratch[sbase + (long)j1 * N + cb * TC + c] = buf[revl[j1] * TC + c];
+            if (cb * TC + c < N) a.scratch[sbase + (long)j1 * N + cb * TC + c] = buf[revl[j1] * TC + c];
         }
     }
 }
