@@ -49,10 +49,11 @@ def test_so3_complex_fp64_b16(fftso3, torch_cuda, oracle):
     assert rel_l2(c[0], F[0]) < FP64_TOL
 
 
-@pytest.mark.parametrize("B", [1, 2, 3, 6, 11, 21, 24, 32, 80])
+@pytest.mark.parametrize("B", [1, 2, 3, 6, 11, 21, 24, 32, 51, 80])
 def test_so3_complex_fp64_sizes(fftso3, torch_cuda, oracle, B):
-    """Bandlimits whose 2B is 2, 4, 6, 12, 22 (direct DFT), 42, 48, 64 and 160 (transposed
-    coefficient staging of the chunked kernels above l_max = 63)."""
+    """Bandlimits whose 2B is 2, 4, 6, 12, 22 (direct DFT), 42, 48, 64, 102 (two-pass FFT with a
+    64-row tile and a partial last block) and 160 (transposed coefficient staging of the chunked
+    kernels above l_max = 63)."""
     F = inputs.so3_coefficients(B, 1, 100 + B)
     g_ref = oracle.so3_inverse(B, F[0])
     g, c = so3_roundtrip(fftso3, torch_cuda, B, F, grid_in=g_ref[None])
