@@ -742,6 +742,21 @@ static bool inv_grouped()
     return on != 0;
 }
 
+// Threads of a whole-slice FFT workgroup: about 8 slice elements per thread (at least N^2 / 16,
+// the direct DFT's register bound), so the small slices of low bandlimits do not tie up
+// 1024-thread workgroups with most threads idle and more of them share a CU.
+// FFTSO3_FUSED_THREADS=0 keeps 1024.
+static int fused_threads(int N)
+{
+    static const int on = [] {
+        const char *e = getenv("FFTSO3_FUSED_THREADS");
+        return e ? atoi(e) : 1;
+    }();
+    if (!on) return kFftThreads;
+    const int t = (N * N / 8 + 63) / 64 * 64;
+    return std::max(64, std::min(kFftThreads, t));
+}
+
 template <int NP>
 int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *in, void *out, hipStream_t s)
 {
@@ -922,8 +937,8 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                 }
             } else if (c->fused) {
                 size_t lds = (size_t)N * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
             } else {
                 a.tile = c->so3_tile;
                 int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
@@ -1011,8 +1026,8 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                 }
             } else if (c->fused) {
                 size_t lds = (size_t)N * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(kFftThreads), lds, s, a);
+                if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
             } else {
                 a.tile = c->so3_tile;
                 int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
