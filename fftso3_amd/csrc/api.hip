@@ -742,10 +742,10 @@ static bool inv_grouped()
     return on != 0;
 }
 
-// Threads of a whole-slice FFT workgroup: about 8 slice elements per thread (at least N^2 / 16,
-// the direct DFT's register bound), so the small slices of low bandlimits do not tie up
-// 1024-thread workgroups with most threads idle and more of them share a CU.
-// FFTSO3_FUSED_THREADS=0 keeps 1024.
+// Threads of a whole-slice FFT workgroup: N^2 / 16 rounded up to whole waves (16 slice elements
+// per thread, the direct DFT's register bound FFT_EPT), so the small slices of low bandlimits do
+// not tie up 1024-thread workgroups with most threads idle and more of them share a CU (measured:
+// 16 elements per thread beat 8, which beat a fixed 1024).  FFTSO3_FUSED_THREADS=0 keeps 1024.
 static int fused_threads(int N)
 {
     static const int on = [] {
@@ -753,7 +753,7 @@ static int fused_threads(int N)
         return e ? atoi(e) : 1;
     }();
     if (!on) return kFftThreads;
-    const int t = (N * N / 8 + 63) / 64 * 64;
+    const int t = (N * N / FFT_EPT + 63) / 64 * 64;
     return std::max(64, std::min(kFftThreads, t));
 }
 
