@@ -848,49 +848,66 @@ template <typename T, int DIR>
 __global__ void so3_tmix(ChunkArgs ca, const void *src, void *dst, long ws_stride)
 {
     typedef typename cplx_t<T>::type T2;
+    // one thread per orbit {(+-a, +-b)} of a degree l (a, b in [0, l]): its (up to) 4 inputs are
+    // read once and its (up to) 4 outputs written; orbits of l start at l (l+1) (2l+1) / 6
     const int B = ca.B;
-    const long nc = so3_ncoef(B);
-    const long total = nc * ca.nfc;
+    const long no = (long)B * (B + 1) * (2 * B + 1) / 6;
+    const long total = no * ca.nfc;
+    auto orb_off = [](int l) { return l * (l + 1) * (2 * l + 1) / 6; };
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-        int fi = (int)(e / nc);
-        int idx = (int)(e - (long)fi * nc);
-        int l = (int)cbrtf((float)idx * 0.75f);
-        while (so3_off(l + 1) <= idx) l++;
-        while (so3_off(l) > idx) l--;
-        int r = idx - so3_off(l), w = 2 * l + 1;
-        int m = r % w - l, n = r / w - l;
+        const int fi = (int)(e / no);
+        const int idx = (int)(e - (long)fi * no);
+        int l = (int)cbrtf((float)idx * 3.0f);
+        while (orb_off(l + 1) <= idx) l++;
+        while (orb_off(l) > idx) l--;
+        const int r = idx - orb_off(l), a = r % (l + 1), b = r / (l + 1);
+        const int na = a ? 2 : 1, nb = b ? 2 : 1;
         const long fo = ca.coef_off[ca.fn[fi]];
         if (DIR == 0) {  // complex workspace -> real output
             const T2 *Fc = reinterpret_cast<const T2 *>(src) + (long)fi * ws_stride;
-            T acc = (T)0;
-            for (int ia = 0; ia < (m ? 2 : 1); ia++) {
-                int av = ia ? -m : m;
-                T2 tm = tmat<T2>(m, av);
-                for (int ib = 0; ib < (n ? 2 : 1); ib++) {
-                    int bv = ib ? -n : n;
-                    T2 tn = tmat<T2>(n, bv);
-                    T2 f = Fc[so3_idx(l, av, bv)];
-                    T2 x = cmul(cmul(tm, f), cmk<T2>(tn.x, -tn.y));
-                    acc += x.x;
+            T2 f[2][2];
+            for (int ia = 0; ia < na; ia++)
+                for (int ib = 0; ib < nb; ib++) f[ia][ib] = Fc[so3_idx(l, ia ? -a : a, ib ? -b : b)];
+            T *out = reinterpret_cast<T *>(dst) + fo;
+            for (int io = 0; io < na; io++) {
+                const int m = io ? -a : a;
+                for (int jo = 0; jo < nb; jo++) {
+                    const int n = jo ? -b : b;
+                    T acc = (T)0;
+                    for (int ia = 0; ia < na; ia++) {
+                        const T2 tm = tmat<T2>(m, ia ? -a : a);
+                        for (int ib = 0; ib < nb; ib++) {
+                            const T2 tn = tmat<T2>(n, ib ? -b : b);
+                            const T2 x = cmul(cmul(tm, f[ia][ib]), cmk<T2>(tn.x, -tn.y));
+                            acc += x.x;
+                        }
+                    }
+                    out[so3_idx(l, m, n)] = acc;
                 }
             }
-            reinterpret_cast<T *>(dst)[fo + idx] = acc;
         } else {         // real input -> complex workspace
             const T *Fr = reinterpret_cast<const T *>(src) + fo;
-            T2 acc = cmk<T2>((T)0, (T)0);
-            // (a,b) = (m,n) here: sum over rows mm in {a,-a}, cols nn in {b,-b}
-            for (int ia = 0; ia < (m ? 2 : 1); ia++) {
-                int mm = ia ? -m : m;
-                T2 tm = tmat<T2>(mm, m);
-                for (int ib = 0; ib < (n ? 2 : 1); ib++) {
-                    int nn = ib ? -n : n;
-                    T2 tn = tmat<T2>(nn, n);
-                    T f = Fr[so3_idx(l, mm, nn)];
-                    T2 x = cmul(cmk<T2>(tm.x * f, -tm.y * f), tn);
-                    acc = cadd(acc, x);
+            T f[2][2];
+            for (int ia = 0; ia < na; ia++)
+                for (int ib = 0; ib < nb; ib++) f[ia][ib] = Fr[so3_idx(l, ia ? -a : a, ib ? -b : b)];
+            T2 *out = reinterpret_cast<T2 *>(dst) + (long)fi * ws_stride;
+            for (int io = 0; io < na; io++) {
+                const int m = io ? -a : a;
+                for (int jo = 0; jo < nb; jo++) {
+                    const int n = jo ? -b : b;
+                    // Fc(m,n) = sum over rows mm in {m,-m}, cols nn in {n,-n} of conj T(mm,m) Fr(mm,nn) T(nn,n)
+                    T2 acc = cmk<T2>((T)0, (T)0);
+                    for (int ia = 0; ia < na; ia++) {
+                        const T2 tm = tmat<T2>(ia ? -a : a, m);
+                        for (int ib = 0; ib < nb; ib++) {
+                            const T2 tn = tmat<T2>(ib ? -b : b, n);
+                            const T fv = f[ia][ib];
+                            acc = cadd(acc, cmul(cmk<T2>(tm.x * fv, -tm.y * fv), tn));
+                        }
+                    }
+                    out[so3_idx(l, m, n)] = acc;
                 }
             }
-            reinterpret_cast<T2 *>(dst)[(long)fi * ws_stride + idx] = acc;
         }
     }
 }
