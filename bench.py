@@ -44,7 +44,7 @@ def alg_bytes(B, n_fn, esz, stage):
     return n_fn * per * esz
 
 
-def cpu_baseline(budget_s=15.0, max_fn=64):
+def cpu_baseline(budget_s=15.0, max_fn=192):
     """The FP64 oracle (grid inverse + forward, OpenMP over beta) on the first functions of the same
     batch, for about budget_s seconds of host time."""
     from oracle import oracle
