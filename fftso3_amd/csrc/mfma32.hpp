@@ -1048,10 +1048,11 @@ __global__ void __launch_bounds__(64 * NW) contract_inv_mxk(MxArgs a)
 // ---------------------------------------------------------------------------------------------
 // Inverse contraction of the chunked path at large bandlimits (FP32, Wigner-d table present):
 // workgroup = (group, 128 beta values); the chunk's coefficient block of the group (M rows x up to
-// 64 columns (function, pair, re/im), pair signs applied) is staged in LDS once, the d table rows
-// stream from HBM as the MFMA A operand (32 consecutive k per row: 128-byte reads), wave w owns
-// beta rows [32w, 32w+32) of the tile.  Columns follow the workspace row order, so each output
-// row is one contiguous segment.
+// 64 columns (function, pair, re/im), pair signs and the SO(3) factor 2l+1 applied) is staged in
+// LDS once, the d table rows stream from HBM as the MFMA A operand (32 consecutive k per row:
+// 128-byte reads, 16 steps per batch with the next batch in flight), wave w owns beta rows
+// [32w, 32w+32) of the tile.  Columns follow the workspace row order, so each output row is one
+// contiguous segment.
 // ---------------------------------------------------------------------------------------------
 // Forward on the d table (chunked path above l_max = 127, FP32): one workgroup per group.  The
 // group's spectrum block (N beta rows x the chunk's <= 64 columns, scaled by w_k) is staged in LDS
