@@ -768,6 +768,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     a.items = (const int *)c->d_items_mx.p;
     a.nitems = c->nitems_mx;
     a.gpw = 1;
+    a.g0 = 0;
     {
         // split each group's column blocks over several workgroups when there are few groups
         const int nfb = (ca.nfc * NP * 2 + 127) / 128;
@@ -804,7 +805,22 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
                 ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
                 const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
                 const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
-                if (ag.gpw == 2 && fwd_mx16()) hipLaunchKernelGGL(contract_fwd_mx16g, dim3(ggrid), dim3(256), glds, s, ag);
+                if (ag.gpw == 2 && fwd_mx16()) {
+                    // groups with at most 32 degrees (m0 >= B - 32; 3/4 of them at l_max = 63) keep
+                    // two 16-row tiles of D in registers instead of four: fewer registers, two
+                    // workgroups per CU
+                    const int gsplit = c->B > 32 ? (c->B - 32) * (c->B - 32) : 0;
+                    MxArgs a4 = ag, a2 = ag;
+                    a4.nitems = gsplit / 2;
+                    a2.g0 = gsplit;
+                    a2.nitems = ag.nitems - gsplit / 2;
+                    if (a4.nitems > 0)
+                        hipLaunchKernelGGL((contract_fwd_mx16g<4>), dim3(deal_grid(a4.nitems * a4.nsplit, a4.nsplit)),
+                                           dim3(256), glds, s, a4);
+                    if (a2.nitems > 0)
+                        hipLaunchKernelGGL((contract_fwd_mx16g<2>), dim3(deal_grid(a2.nitems * a2.nsplit, a2.nsplit)),
+                                           dim3(256), glds, s, a2);
+                }
                 else if (ag.gpw == 2) hipLaunchKernelGGL((contract_fwd_mx32g<2>), dim3(ggrid), dim3(256), glds, s, ag);
                 else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
             } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
@@ -1111,7 +1127,8 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32g<2>));
     ALLOW_LDS((contract_fwd_mx32g<4>));
     ALLOW_LDS((contract_inv_mx32g<2>));
-    ALLOW_LDS(contract_fwd_mx16g);
+    ALLOW_LDS((contract_fwd_mx16g<4>));
+    ALLOW_LDS((contract_fwd_mx16g<2>));
     ALLOW_LDS((contract_fwd_tab<4>));
     ALLOW_LDS((contract_fwd_tab<2>));
     ALLOW_LDS((contract_fwd_tab2<4>));

@@ -35,7 +35,8 @@ struct MxArgs {
     const int *items;        // groups (largest first)
     int nitems;
     int nsplit;              // workgroups per group, each taking a contiguous range of column blocks
-    int gpw;                 // groups per workgroup (contract_fwd_mx32g: items are sets of gpw groups)
+    int gpw;                 // groups per workgroup (grouped kernels: items are sets of gpw groups)
+    int g0;                  // grouped kernels: first group of item 0
     const void *src;         // forward: spectrum workspace; inverse: PT
     void *dst;               // forward: PT; inverse: spectrum workspace
     float *dummy;            // 16-byte slot that absorbs the stores of lanes with nothing to store
@@ -52,7 +53,7 @@ __device__ __forceinline__ bool mx_unit(const MxArgs &a, int nfb, int &g, int &f
     const int unit = xcd_deal(a.gpw > 1 ? a.nsplit : a.nsplit << a.ca.gil);
     if (unit >= n) return false;
     const int it = unit / a.nsplit, part = unit - it * a.nsplit;
-    g = a.gpw > 1 ? it * a.gpw : a.items[it];
+    g = a.gpw > 1 ? a.g0 + it * a.gpw : a.items[it];
     fb0 = (int)((long)part * nfb / a.nsplit);
     fb1 = (int)((long)(part + 1) * nfb / a.nsplit);
     return fb0 < fb1;
@@ -389,7 +390,8 @@ typedef float f32x4m_t __attribute__((ext_vector_type(4)));
 // (44% of the groups at l_max = 63).  Tile layouts (lane i): A[i%16][i/16] = D(l, k), B[i/16][i%16]
 // = spectrum(k, column), result row 4(i/16) + j, column i%16.  Wave w takes group w / 2 and pairs
 // 2(w % 2), 2(w % 2) + 1: one 16-column tile per pair (8 functions, re/im).
-__global__ void __launch_bounds__(256) contract_fwd_mx16g(MxArgs a)
+template <int NRT>
+__global__ void __launch_bounds__(256, NRT == 2 ? 2 : 1) contract_fwd_mx16g(MxArgs a)
 {
     constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -413,18 +415,18 @@ __global__ void __launch_bounds__(256) contract_fwd_mx16g(MxArgs a)
     auto lcol = [](int fl, int gq_, int p) { return (gq_ * WPG + p / GPW) * 32 + (p % GPW) * FB * 2 + fl * 2; };
     const int lc0 = lcol(fl_q, gq, 2 * (q & 1)), lc1 = lcol(fl_q, gq, 2 * (q & 1) + 1);
     const float *S = reinterpret_cast<const float *>(a.src);
-    long roff[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, gb, min(r0 + 8 * i, N - 1)) * CW + 4 * q;
+    // rows r0 + 8i (< N = 128): the workspace rows are [k][g] (no k blocking with the interleave),
+    // so their offsets step by a constant
+    const long rbase = ws_row(ca, gb, r0) * CW + 4 * q, rstep = (ws_row(ca, gb, r0 + 8) - ws_row(ca, gb, r0)) * CW;
     float4 pf[16];
     auto load_block = [&](int fb) {
         const long co = fb * FB + fl_q < ca.nfc ? (long)fb * 128 : 0;
 #pragma unroll
-        for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + roff[i] + co);
+        for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + rbase + i * rstep + co);
     };
     load_block(fb0);
     // D of the two groups: threads 0-127 recurse for the first, 128-255 the second
-    float dR[4][32];
+    float dR[NRT][32];
     {
         const int h = tid >> 7;
         int sm0, sn0;
@@ -442,7 +444,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mx16g(MxArgs a)
         __syncthreads();
         const float *D0 = Dst + gg * 64 * MX_DPITCH + cx * MX_DPITCH + kq;
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < NRT; t++)
 #pragma unroll
             for (int s = 0; s < 32; s++) dR[t][s] = t < nrt ? D0[16 * t * MX_DPITCH + 4 * s] : 0.f;
         __syncthreads();                                 // staging read before it is overwritten
@@ -464,9 +466,9 @@ __global__ void __launch_bounds__(256) contract_fwd_mx16g(MxArgs a)
         }
         __syncthreads();
         if (fb + 1 < fb1) load_block(fb + 1);
-        f32x4m_t acc[4][2];
+        f32x4m_t acc[NRT][2];
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < NRT; t++)
 #pragma unroll
             for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -492,14 +494,14 @@ __global__ void __launch_bounds__(256) contract_fwd_mx16g(MxArgs a)
             }
         };
         if (nrt == 1) run(std::integral_constant<int, 1>());
-        else if (nrt == 2) run(std::integral_constant<int, 2>());
-        else if (nrt == 3) run(std::integral_constant<int, 3>());
-        else run(std::integral_constant<int, 4>());
+        else if (NRT == 2 || nrt == 2) run(std::integral_constant<int, 2>());
+        else if (nrt == 3) run(std::integral_constant<int, NRT >= 3 ? 3 : 2>());
+        else run(std::integral_constant<int, NRT>());
         const int fn = fb * FB + fo;
         const bool okf = fn < ca.nfc;
         float *o = PT + 2 * fn + (cx & 1);
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < NRT; t++) {
             if (t < nrt) {
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
