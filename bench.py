@@ -24,7 +24,8 @@ HBM_PEAK_GBS = 8000.0
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "so3_b64_batch256_fp32_pmc.json")
 # kernels of the batch-wide FP32 path at l_max = 63 with its default schedule (group interleave 1:
 # paired-function FFTs, contractions over the two groups that share a workspace line), by stage
-STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd2<false>", "contract_fwd": "contract_fwd_mx16g",
+# (the forward contraction is two launches, <4> for the groups with more than 32 degrees and <2>)
+STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd2<false>", "contract_fwd": "contract_fwd_mx16g<4>",
                 "contract_inv": "contract_inv_mx32g<2>", "fft_inv": "so3_fft128_inv2<false>"}
 
 
