@@ -134,20 +134,19 @@ __device__ __forceinline__ void cols_pass(float2 *buf, const float2 *twl, int ti
     __syncthreads();
 }
 
-// 2-D forward DFT of slice (k, fi) into LDS (so3_fft128_fwd, so3_fft128_fwd2)
+// 2-D forward DFT of slice (k, fi) into LDS (so3_fft128_fwd, so3_fft128_fwd2), in three parts:
+// the grid loads into registers, row stage A into LDS, and the remaining passes.
+// This thread's two stage A butterflies.  Complex: row tid / 8, columns j0 + 16r and j0 + 1 + 16r
+// (j0 = 2 (tid % 8)), so each 16-byte load brings both butterflies' inputs of one r (measured
+// 1.97 -> 1.89 ms at l_max = 63 x 256).  Real: rows tid / 16 and 64 + tid / 16, column tid % 16
+// (4-byte loads).
 template <bool REAL>
-__device__ __forceinline__ void fft128_fwd_slice(const FftArgs<float> &a, float2 *buf, const float2 *twl, int k,
-                                                 int fi, int tid)
+__device__ __forceinline__ void fft128_load(const FftArgs<float> &a, int k, int fi, int tid, float2 (&v)[2][8])
 {
     const ChunkArgs &ca = a.ca;
     const long base = ca.grid_off[ca.fn[fi]] + (long)k * F128 * F128;
-    // this thread's two stage A butterflies.  Complex: row tid / 8, columns j0 + 16r and
-    // j0 + 1 + 16r (j0 = 2 (tid % 8)), so each 16-byte load brings both butterflies' inputs of one
-    // r (measured 1.97 -> 1.89 ms at l_max = 63 x 256).  Real: rows tid / 16 and 64 + tid / 16,
-    // column tid % 16 (4-byte loads).
     const int row0 = REAL ? tid >> 4 : tid >> 3, j0 = REAL ? tid & 15 : 2 * (tid & 7);
-    const int drow = REAL ? 64 : 0, dj = REAL ? 0 : 1;
-    float2 v[2][8];
+    const int drow = REAL ? 64 : 0;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         if (REAL) {
@@ -161,6 +160,13 @@ __device__ __forceinline__ void fft128_fwd_slice(const FftArgs<float> &a, float2
             v[1][r] = make_float2(x.z, x.w);
         }
     }
+}
+
+template <bool REAL>
+__device__ __forceinline__ void fft128_stageA_lds(float2 (&v)[2][8], float2 *buf, const float2 *twl, int tid)
+{
+    const int row0 = REAL ? tid >> 4 : tid >> 3, j0 = REAL ? tid & 15 : 2 * (tid & 7);
+    const int drow = REAL ? 64 : 0, dj = REAL ? 0 : 1;
     __syncthreads();  // twiddle table / previous slice's LDS reads
 #pragma unroll
     for (int i = 0; i < 2; i++) {
@@ -170,9 +176,23 @@ __device__ __forceinline__ void fft128_fwd_slice(const FftArgs<float> &a, float2
         for (int s = 0; s < 8; s++) buf[row * P128 + j + 16 * s] = v[i][s];
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ void fft128_rest(float2 *buf, const float2 *twl, int tid)
+{
     rows_stageB(buf, tid);
     __syncthreads();
     cols_pass(buf, twl, tid);
+}
+
+template <bool REAL>
+__device__ __forceinline__ void fft128_fwd_slice(const FftArgs<float> &a, float2 *buf, const float2 *twl, int k,
+                                                 int fi, int tid)
+{
+    float2 v[2][8];
+    fft128_load<REAL>(a, k, fi, tid, v);
+    fft128_stageA_lds<REAL>(v, buf, twl, tid);
+    fft128_rest(buf, twl, tid);
 }
 
 // the 4 pairs of group g of the spectrum in LDS, as two float4 (one 32-byte record)
@@ -418,7 +438,14 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
     const bool two = fi + 1 < ca.nfc;
     const int tid = threadIdx.x;
     if (tid < F128) twl[tid] = a.tw[tid];
-    fft128_fwd_slice<REAL>(a, buf, twl, k, fi, tid);
+    float2 vA[2][8], vB[2][8];
+    fft128_load<REAL>(a, k, fi, tid, vA);
+    fft128_stageA_lds<REAL>(vA, buf, twl, tid);
+    // the second slice's loads go out once the first slice's inputs are in LDS, and are in flight
+    // during its remaining passes (1.89 -> 1.83 ms at l_max = 63 x 256; issuing them together with
+    // the first slice's loads was slower)
+    if (two) fft128_load<REAL>(a, k, fi + 1, tid, vB);
+    fft128_rest(buf, twl, tid);
     float4 r0[4][2];
 #pragma unroll
     for (int i = 0; i < 4; i++) fft128_record(buf, tid + i * 1024, r0[i][0], r0[i][1]);
@@ -432,7 +459,8 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
         }
         return;
     }
-    fft128_fwd_slice<REAL>(a, buf, twl, k, fi + 1, tid);
+    fft128_stageA_lds<REAL>(vB, buf, twl, tid);
+    fft128_rest(buf, twl, tid);
     int tl = tid;
     asm volatile("" : "+v"(tl));  // decode the groups again rather than keep them live across the second slice
 #pragma unroll
