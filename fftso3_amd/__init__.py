@@ -44,6 +44,7 @@ _SIGS = {
     "fftso3_inverse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fftso3_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "fftso3_last_error": (ctypes.c_char_p, []),
+    "fftso3_debug_fail_alloc": (ctypes.c_int, [ctypes.c_int]),
     "fftso3_timing_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_timing_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_long)]),

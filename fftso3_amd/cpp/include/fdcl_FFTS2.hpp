@@ -61,7 +61,11 @@ protected:
     int B = 1;
     int precision_ = FFTSO3_FP64;
     std::vector<double> compute_weight();
-    std::shared_ptr<detail::DevicePlan> dev_ = std::make_shared<detail::DevicePlan>();
+    detail::DeviceHandle dev_;
+
+public:
+    // identity of this object's device plan (distinct for every object, copies included)
+    const void *device_plan_id() const { return dev_.get(); }
 };
 
 class FFTS2_real : public FFTS2_complex

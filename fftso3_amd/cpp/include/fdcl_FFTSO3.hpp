@@ -102,6 +102,26 @@ private:
     size_t pin_bytes_ = 0;
     void *stream_ = nullptr, *ev_[2] = {nullptr, nullptr};
 };
+
+// The device plan of one transform object.  Copies never share it: a copied (or assigned)
+// object starts with a device plan of its own, created on first use, so copies may run from
+// different threads just as the reference's by-value objects can.
+class DeviceHandle
+{
+public:
+    DeviceHandle() : p_(new DevicePlan) {}
+    DeviceHandle(const DeviceHandle &) : p_(new DevicePlan) {}
+    DeviceHandle &operator=(const DeviceHandle &o)
+    {
+        if (this != &o) p_.reset(new DevicePlan);
+        return *this;
+    }
+    DevicePlan *operator->() const { return p_.get(); }
+    const DevicePlan *get() const { return p_.get(); }
+
+private:
+    std::unique_ptr<DevicePlan> p_;
+};
 }  // namespace detail
 
 class FFTSO3_complex
@@ -190,7 +210,11 @@ protected:
     std::vector<double> compute_weight();
     FFTSO3_matrix_real wigner_d(double beta, int L);
     FFTSO3_matrix_complex wigner_D(double alpha, double beta, double gamma, int L);
-    std::shared_ptr<detail::DevicePlan> dev_ = std::make_shared<detail::DevicePlan>();
+    detail::DeviceHandle dev_;
+
+public:
+    // identity of this object's device plan (distinct for every object, copies included)
+    const void *device_plan_id() const { return dev_.get(); }
 };
 
 class FFTSO3_real : public FFTSO3_complex

@@ -43,24 +43,39 @@ constexpr int kMaxLds = 160 * 1024;  // LDS per workgroup on gfx950
 constexpr int kFftThreads = 1024;
 constexpr int kNumStages = 6;
 
-// device buffer owned by a plan
+// fftso3_debug_fail_alloc: the countdown to the device allocation of this thread that fails
+thread_local int g_fail_alloc_after = 0;
+
+// device buffer owned by a plan; n is the size actually held (0 when nothing is)
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
-    ~DevBuf()
+    ~DevBuf() { release(); }
+    void release()
     {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
     }
     int alloc(size_t bytes)
     {
-        n = bytes;
+        release();
         if (bytes == 0) return FFTSO3_OK;
+        if (g_fail_alloc_after > 0 && --g_fail_alloc_after == 0)
+            return fail(FFTSO3_ENOMEM, "hipMalloc(%zu) failed (forced by fftso3_debug_fail_alloc)", bytes);
         if (hipMalloc(&p, bytes) != hipSuccess) {
             p = nullptr;
             (void)hipGetLastError();
             return fail(FFTSO3_ENOMEM, "hipMalloc(%zu) failed", bytes);
         }
+        n = bytes;
         return FFTSO3_OK;
+    }
+    // hold exactly `bytes` (keeps the current buffer when it already has that size)
+    int resize(size_t bytes)
+    {
+        if (n == bytes && (p || bytes == 0)) return FFTSO3_OK;
+        return alloc(bytes);
     }
     template <typename V> int upload(const std::vector<V> &v)
     {
@@ -113,7 +128,9 @@ struct fftso3_plan_s {
     DevBuf d_fc;        // complex coefficients of the real path / transposed coefficients PT
     DevBuf d_dummy;     // store sink of the batch-wide contraction (lanes with nothing to store)
     DevBuf d_tmp;       // packed complex coefficients of the batch-wide real path
-    size_t ws_bytes = 0, scratch_bytes = 0, fc_bytes = 0, tmp_bytes = 0;
+    // false while the plan's tables and workspaces are not all in place (a failed finalize_plan,
+    // e.g. out of memory in set_chunk or autotune): transforms refuse to run on such a plan
+    bool valid = false;
     int klo = 0;                       // row blocking of the batch-wide workspace (ws_row)
     int gil = 1;                       // group interleave of the batch-wide workspace (ws_rec)
     int split_mul = 1;                 // multiplier of the automatic group split (batch-wide kernels)
@@ -416,6 +433,7 @@ static int ws_klo()
 int finalize_plan(fftso3_plan_s *pl)
 {
     pl->drop_graphs();  // captured graphs hold the old buffers
+    pl->valid = false;
     int st;
     for (auto *c : pl->classes) delete c;
     pl->classes.clear();
@@ -453,15 +471,10 @@ int finalize_plan(fftso3_plan_s *pl)
     // (clamped columns of a partial block)
     ws += 4096;
     if (fc) fc += 512;
-    if (!pl->d_dummy.p && (st = pl->d_dummy.alloc(256))) return st;
-    if (ws != pl->ws_bytes) { pl->d_ws.~DevBuf(); new (&pl->d_ws) DevBuf(); if ((st = pl->d_ws.alloc(ws))) return st; }
-    if (scr != pl->scratch_bytes) { pl->d_scratch.~DevBuf(); new (&pl->d_scratch) DevBuf(); if ((st = pl->d_scratch.alloc(scr))) return st; }
-    if (fc != pl->fc_bytes) { pl->d_fc.~DevBuf(); new (&pl->d_fc) DevBuf(); if ((st = pl->d_fc.alloc(fc))) return st; }
-    if (tmpb != pl->tmp_bytes) { pl->d_tmp.~DevBuf(); new (&pl->d_tmp) DevBuf(); if ((st = pl->d_tmp.alloc(tmpb))) return st; }
-    pl->tmp_bytes = tmpb;
-    pl->ws_bytes = ws;
-    pl->scratch_bytes = scr;
-    pl->fc_bytes = fc;
+    if ((st = pl->d_dummy.resize(256)) || (st = pl->d_ws.resize(ws)) || (st = pl->d_scratch.resize(scr)) ||
+        (st = pl->d_fc.resize(fc)) || (st = pl->d_tmp.resize(tmpb)))
+        return st;
+    pl->valid = true;
     return FFTSO3_OK;
 }
 
@@ -1258,7 +1271,7 @@ int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk)
         return fail(FFTSO3_EINVAL, "chunk must be 1, 2, 4 or 8 (got %d)", chunk);
     if ((plan->prec == FFTSO3_BF16X3 || plan->prec == FFTSO3_BF16X6) && chunk != 8)
         return fail(FFTSO3_EINVAL, "split-BF16 plans process 8 functions per chunk");
-    if (chunk == plan->C && plan->force_chunked) return FFTSO3_OK;
+    if (chunk == plan->C && plan->force_chunked && plan->valid) return FFTSO3_OK;
     plan->C = chunk;
     plan->force_chunked = true;
     return finalize_plan(plan);
@@ -1305,12 +1318,20 @@ long fftso3_plan_coef_elems(fftso3_plan_t plan) { return plan ? plan->coef_total
 long fftso3_plan_workspace_bytes(fftso3_plan_t plan)
 {
     if (!plan) return -1;
-    return (long)(plan->ws_bytes + plan->scratch_bytes + plan->fc_bytes + plan->tmp_bytes);
+    return (long)(plan->d_ws.n + plan->d_scratch.n + plan->d_fc.n + plan->d_tmp.n);
+}
+
+static int check_valid(fftso3_plan_t plan)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (!plan->valid)
+        return fail(FFTSO3_EINVAL, "plan is not usable: its last reconfiguration failed (%s)", "set_chunk or autotune");
+    return FFTSO3_OK;
 }
 
 static int run_dispatch(fftso3_plan_t plan, bool fwd, const void *in, void *out, void *stream)
 {
-    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (int st = check_valid(plan)) return st;
     if (!in || !out) return fail(FFTSO3_EINVAL, "NULL device pointer");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     auto eager = [&]() {
@@ -1387,7 +1408,7 @@ static int evaluate(fftso3_plan_s *pl, int fn, const void *coef, const double *a
 int fftso3_evaluate(fftso3_plan_t plan, int fn, const void *d_coef, const double *d_angles, long n_points,
                     void *d_values, void *stream)
 {
-    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (int st = check_valid(plan)) return st;
     if (fn < 0 || fn >= plan->n_fn) return fail(FFTSO3_EINVAL, "function index %d outside [0, %d)", fn, plan->n_fn);
     if (n_points < 0) return fail(FFTSO3_EINVAL, "n_points must be >= 0 (got %ld)", n_points);
     if (n_points == 0) return FFTSO3_OK;
@@ -1410,7 +1431,7 @@ int fftso3_plan_set_graph(fftso3_plan_t plan, int on)
 // chunked kernels the chunk size.  The two arrays serve as scratch.
 int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps)
 {
-    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (int st = check_valid(plan)) return st;
     if (!d_grid || !d_coef) return fail(FFTSO3_EINVAL, "NULL device pointer");
     if (reps < 1 || reps > 100) return fail(FFTSO3_EINVAL, "reps must be in [1, 100] (got %d)", reps);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -1491,6 +1512,13 @@ const char *fftso3_status_string(int status)
 }
 
 const char *fftso3_last_error(void) { return g_last_error.c_str(); }
+
+int fftso3_debug_fail_alloc(int after)
+{
+    if (after < 0) return fail(FFTSO3_EINVAL, "after must be >= 0 (got %d)", after);
+    g_fail_alloc_after = after;
+    return FFTSO3_OK;
+}
 
 int fftso3_timing_enable(fftso3_plan_t plan, int on)
 {

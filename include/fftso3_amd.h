@@ -133,6 +133,12 @@ int fftso3_evaluate(fftso3_plan_t plan, int fn, const void *d_coef, const double
 const char *fftso3_status_string(int status);
 const char *fftso3_last_error(void);
 
+/* Error-path testing: the after-th device allocation made by the calling thread from now on
+ * fails with FFTSO3_ENOMEM (after = 0 switches this off).  A plan whose reconfiguration failed
+ * (set_chunk, autotune) refuses to transform until a later reconfiguration succeeds.  No
+ * counterpart in the reference. */
+int fftso3_debug_fail_alloc(int after);
+
 /* Per-stage kernel timing with HIP events on the launch stream.  Stages:
  * 0 fft(forward) 1 contraction(forward) 2 basis(forward) 3 basis(inverse) 4 contraction(inverse)
  * 5 fft(inverse).  read() synchronises, adds the elapsed milliseconds and launch counts since the

@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "fdcl_FFTS2.hpp"
 #include "fdcl_FFTSO3.hpp"
@@ -114,6 +116,21 @@ static int cpu_checks()
         const auto DE = fc.wigner_D(0.3, b, 1.9);
         expect("wigner_D_rotation_overload", (DR - DE).norm(), 1e-12);
     }
+    // copies never share a device plan (the reference's objects are independent values)
+    {
+        fdcl::FFTSO3_real a(6);
+        fdcl::FFTSO3_real b(a);
+        fdcl::FFTSO3_real c(3);
+        c = a;
+        std::vector<fdcl::FFTSO3_complex> v(3, fdcl::FFTSO3_complex(5));
+        fdcl::FFTS2_real s(4);
+        fdcl::FFTS2_real t(s);
+        const bool distinct = a.device_plan_id() != b.device_plan_id() && a.device_plan_id() != c.device_plan_id() &&
+                              b.device_plan_id() != c.device_plan_id() && v[0].device_plan_id() != v[1].device_plan_id() &&
+                              v[1].device_plan_id() != v[2].device_plan_id() && s.device_plan_id() != t.device_plan_id();
+        expect("copies_have_own_device_plans", distinct ? 0.0 : 1.0, 0);
+        expect("copy_keeps_l_max", (double)(b.l_max != 6) + (double)(c.l_max != 6) + (double)(t.l_max != 4), 0);
+    }
     // pointwise real inverse equals the real part of the complex inverse of T^H F_r T
     {
         const int L = 4;
@@ -192,6 +209,38 @@ static int gpu_checks()
         fc.set_precision(FFTSO3_FP32);
         const auto F3 = fc.forward_transform(fc.inverse_transform(F));
         expect("so3_fp32_round_trip", rel(F3.v, F.v), 2e-6);
+    }
+    // copies of one object used from two threads at once: each copy has its own device plan, so
+    // both give the result of the original (run alone) with nothing shared between them
+    {
+        const int L = 14;
+        fdcl::FFTSO3_real proto(L);
+        fdcl::FFTSO3_matrix_real F1(L), F2(L);
+        F1.setRandom();
+        F2.setRandom();
+        const auto ref1 = proto.forward_transform(proto.inverse_transform(F1));
+        const auto ref2 = proto.forward_transform(proto.inverse_transform(F2));
+        std::vector<fdcl::FFTSO3_real> copies(2, proto);
+        std::vector<double> errs(2, 1.0);
+        std::vector<std::thread> th;
+        for (int t = 0; t < 2; t++)
+            th.emplace_back([&, t]() {
+                try {
+                    double e = 0;
+                    for (int r = 0; r < 6; r++) {
+                        const auto &F = (r + t) % 2 ? F2 : F1;
+                        const auto &ref = (r + t) % 2 ? ref2 : ref1;
+                        const auto out = copies[t].forward_transform(copies[t].inverse_transform(F));
+                        e = std::max(e, rel(out.v, ref.v));
+                    }
+                    errs[t] = e;
+                } catch (const std::exception &ex) {
+                    std::printf("FAIL thread %d: %s\n", t, ex.what());
+                    errs[t] = 1.0;
+                }
+            });
+        for (auto &x : th) x.join();
+        expect("copies_from_two_threads", std::max(errs[0], errs[1]), 1e-15);
     }
     // batched pointwise evaluation on the GPU against the host pointwise inverse
     {

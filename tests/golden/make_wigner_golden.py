@@ -63,6 +63,20 @@ def main():
     for beta in (float(mp.mpf(511) * mp.pi / 1024), float(mp.mpf(1) * mp.pi / 1024)):
         for (l, m, n) in [(255, 255, -255), (255, -255, 255), (200, 150, -140), (255, 0, 0), (128, 128, 128)]:
             samples.append([l, m, n, beta, float(wigner_d(l, m, n, beta))])
+    # large degrees in the oscillatory region, where a lost term or a wrong sign changes the value
+    # at O(1): 48 samples with 128 <= l <= 255 and |d| > 1e-3 (rejection sampling, own stream so
+    # the samples above stay as they were)
+    big = random.Random(20261204)
+    nbig = 0
+    while nbig < 48:
+        l = big.randint(128, 255)
+        m = big.randint(-l, l)
+        n = big.randint(-l, l)
+        beta = big.uniform(0.05, 3.09)
+        v = wigner_d(l, m, n, beta)
+        if abs(v) > mp.mpf("1e-3"):
+            samples.append([l, m, n, beta, float(v)])
+            nbig += 1
     with open(os.path.join(HERE, "wigner_d.json"), "w") as f:
         json.dump({"source": "Wigner explicit sum, mpmath dps=250", "fields": ["l", "m", "n", "beta", "d"],
                    "samples": samples}, f, indent=0)
@@ -74,7 +88,8 @@ def main():
             m = rnd.randint(0, l)
             leg.append([l, m, x, float(legendre_norm(l, m, x))])
     with open(os.path.join(HERE, "legendre.json"), "w") as f:
-        json.dump({"source": "sqrt((2l+1)/4pi (l-m)!/(l+m)!) * mpmath.legenp, dps=250",
+        json.dump({"source": "sqrt((2l+1)/4pi (l-m)!/(l+m)!) * (-1)^m (1-x^2)^(m/2) d^m/dx^m P_l(x), "
+                             "P_l from its explicit polynomial, mpmath dps=250",
                    "fields": ["l", "m", "x", "nP"], "samples": leg}, f, indent=0)
 
 

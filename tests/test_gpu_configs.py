@@ -51,28 +51,34 @@ def test_uneven_real_fp32_batch(fftso3, torch_cuda, oracle):
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
 def test_so3_b256(fftso3, torch_cuda, oracle, prec, tol):
-    """Config 5a shape: l_max=255 (512^3 grid), one function of the seed-4 workload.  Round trip on
-    the GPU, and the grid at 12 sampled points against the oracle's pointwise inverse."""
+    """Config 5a: l_max=255 (512^3 grids), all 8 functions of the seed-4 workload.  Every function
+    round trips on the GPU; the grids of the first and the last function at 12 sampled points each
+    against the oracle's pointwise inverse."""
     torch = torch_cuda
-    B = 256
-    F = inputs.workload_b256(1)
+    B, n = 256, 8
+    F = inputs.workload_b256(n)
     print("b256 %s: plan" % prec, flush=True)
-    plan = fftso3.Plan(B - 1, 1, "so3", "complex", prec)
+    plan = fftso3.Plan(B - 1, n, "so3", "complex", prec)
     gdt, cdt = plan.dtypes()
     dF = torch.from_numpy(F.reshape(-1)).to(cdt).cuda()
     g = plan.inverse(dF)
     c = plan.forward(g)
     torch.cuda.synchronize()
     print("b256 %s: transforms done" % prec, flush=True)
-    assert rel_l2(c.cpu().numpy().astype(np.complex128), F[0]) < tol
+    a = c.reshape(n, -1).to(torch.complex128)
+    b = dF.reshape(n, -1).to(torch.complex128)
+    err = (torch.linalg.norm(a - b, dim=1) / torch.linalg.norm(b, dim=1)).cpu().numpy()
+    print("b256 %s: round trip per function %s" % (prec, np.array2string(err, precision=2)), flush=True)
+    assert float(err.max()) < tol
     N = 2 * B
     rng = np.random.default_rng(256)
-    pts = rng.integers(0, N, size=(12, 3))
-    grid = g.reshape(N, N, N)
-    got = np.array([complex(grid[k, j1, j2].item()) for k, j1, j2 in pts])
-    ref = np.array([oracle.so3_inverse_point(B, F[0], j1 * np.pi / B, (2 * k + 1) * np.pi / (4 * B),
-                                             j2 * np.pi / B) for k, j1, j2 in pts])
-    assert rel_l2(got, ref) < tol
+    grid = g.reshape(n, N, N, N)
+    for i in (0, n - 1):
+        pts = rng.integers(0, N, size=(12, 3))
+        got = np.array([complex(grid[i, k, j1, j2].item()) for k, j1, j2 in pts])
+        ref = np.array([oracle.so3_inverse_point(B, F[i], j1 * np.pi / B, (2 * k + 1) * np.pi / (4 * B),
+                                                 j2 * np.pi / B) for k, j1, j2 in pts])
+        assert rel_l2(got, ref) < tol, i
 
 
 def test_s2_b128_batch4096_fp32(fftso3, torch_cuda, oracle):

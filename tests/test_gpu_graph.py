@@ -58,3 +58,39 @@ def test_autotune_keeps_parity(fftso3, torch_cuda, oracle):
         ref = oracle.so3_inverse(B, F[i]).reshape(-1)
         assert np.linalg.norm(g[i] - ref) / np.linalg.norm(ref) < 2e-6
         assert np.linalg.norm(c[i] - F[i]) / np.linalg.norm(F[i]) < 2e-6
+
+
+def test_failed_reconfiguration_leaves_plan_unusable(fftso3, torch_cuda, oracle):
+    """An allocation that fails inside set_chunk leaves the plan refusing to transform (EINVAL,
+    no launch) instead of running kernels on freed buffers; a later successful set_chunk
+    restores it, and its results meet the FP64 bar again."""
+    import ctypes
+    torch = torch_cuda
+    L = fftso3.lib()
+    B, n = 12, 5
+    F = inputs.so3_coefficients(B, n, 4321)
+    plan = fftso3.Plan(B - 1, n, "so3", "complex", "fp64")
+    dF = torch.from_numpy(F.reshape(-1)).to(torch.complex128).cuda()
+    g0 = plan.inverse(dF)
+    torch.cuda.synchronize()
+    assert L.fftso3_debug_fail_alloc(1) == 0
+    with pytest.raises(fftso3.FFTSO3Error) as e:
+        plan.set_chunk(2)
+    assert e.value.status.startswith("FFTSO3_ENOMEM")
+    assert L.fftso3_debug_fail_alloc(0) == 0
+    g = torch.full_like(g0, 7.0)
+    st = L.fftso3_inverse(plan.handle, ctypes.c_void_p(dF.data_ptr()), ctypes.c_void_p(g.data_ptr()), None)
+    assert st == 1 and b"not usable" in L.fftso3_last_error()
+    torch.cuda.synchronize()
+    assert bool((g == 7.0).all()), "a kernel ran on the unusable plan"
+    plan.set_chunk(2)
+    g = plan.inverse(dF)
+    c = plan.forward(g)
+    torch.cuda.synchronize()
+    g = g.cpu().numpy().reshape(n, -1)
+    c = c.cpu().numpy().reshape(n, -1)
+    for i in (0, n - 1):
+        ref = oracle.so3_inverse(B, F[i]).reshape(-1)
+        assert np.linalg.norm(g[i] - ref) / np.linalg.norm(ref) < 1e-10
+        assert np.linalg.norm(c[i] - F[i]) / np.linalg.norm(F[i]) < 1e-10
+    plan.close()
