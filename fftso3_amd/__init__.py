@@ -42,6 +42,7 @@ _SIGS = {
     "fftso3_plan_workspace_bytes": (ctypes.c_long, [ctypes.c_void_p]),
     "fftso3_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fftso3_inverse": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fftso3_inverse_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fftso3_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "fftso3_last_error": (ctypes.c_char_p, []),
     "fftso3_debug_fail_alloc": (ctypes.c_int, [ctypes.c_int]),

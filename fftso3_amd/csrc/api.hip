@@ -86,6 +86,31 @@ struct DevBuf {
     }
 };
 
+// Device memory of fftso3_inverse_noalloc: the spectrum workspace and coefficient staging live
+// here instead of in buffers of the plan.  Every GPU that loads the library holds its own copy
+// until the process ends; host code must never pass the symbol itself to a kernel or a copy (on
+// the host it is only a stand-in), only the per-device address from hipGetSymbolAddress.
+constexpr size_t kNoallocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Looks up this device's address of `scratch` (once per device, with the device current).
+int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(FFTSO3_EUNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+    if (g_scratch_done[dev]) return FFTSO3_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(FFTSO3_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+    g_scratch_done[dev] = true;
+    return FFTSO3_OK;
+}
+
 // One bandlimit within a plan
 struct Class {
     int B = 0, N = 0;
@@ -131,6 +156,7 @@ struct fftso3_plan_s {
     // false while the plan's tables and workspaces are not all in place (a failed finalize_plan,
     // e.g. out of memory in set_chunk or autotune): transforms refuse to run on such a plan
     bool valid = false;
+    int device = 0;                    // the HIP device current when the plan was created
     int klo = 0;                       // row blocking of the batch-wide workspace (ws_row)
     int gil = 1;                       // group interleave of the batch-wide workspace (ws_rec)
     int split_mul = 1;                 // multiplier of the automatic group split (batch-wide kernels)
@@ -1194,8 +1220,12 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
             return fail(FFTSO3_EINVAL, "l_max[%d] = %d outside [0, %d]", i, lmax[i], maxB - 1);
     int st;
     if ((st = set_kernel_attrs())) return st;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if ((st = scratch_setup(dev))) return st;
     fftso3_plan_s *pl = new (std::nothrow) fftso3_plan_s();
     if (!pl) return fail(FFTSO3_ENOMEM, "host allocation failed");
+    pl->device = dev;
     pl->domain = domain;
     pl->kind = kind;
     pl->prec = precision;
@@ -1497,6 +1527,43 @@ int fftso3_forward(fftso3_plan_t plan, const void *d_grid, void *d_coef, void *s
 int fftso3_inverse(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream)
 {
     return run_dispatch(plan, false, d_coef, d_grid, stream);
+}
+
+int fftso3_inverse_noalloc(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream)
+{
+    if (int st = check_valid(plan)) return st;
+    if (!d_coef || !d_grid) return fail(FFTSO3_EINVAL, "NULL device pointer");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != plan->device)
+        return fail(FFTSO3_EINVAL, "plan was created on device %d, device %d is current", plan->device, dev);
+    if (dev < 0 || dev >= kMaxDevices || !g_scratch_done[dev]) return fail(FFTSO3_EHIP, "no scratch address for device %d", dev);
+    // the plan's workspaces, each rounded up to 256 bytes, laid out one after the other in scratch
+    DevBuf *bufs[4] = {&plan->d_ws, &plan->d_scratch, &plan->d_fc, &plan->d_tmp};
+    size_t off[4], total = 0;
+    for (int i = 0; i < 4; i++) {
+        off[i] = total;
+        total += (bufs[i]->n + 255) & ~(size_t)255;
+    }
+    if (total > kNoallocScratchBytes)
+        return fail(FFTSO3_EINVAL, "this plan's inverse needs %zu bytes of workspace; the static scratch holds %zu (%d MiB)",
+                    total, kNoallocScratchBytes, (int)(kNoallocScratchBytes >> 20));
+    // run the plan's own launch sequence eagerly (no graph cache, no timing events) with its
+    // workspace pointers redirected into scratch; restored before returning
+    unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+    void *saved[4];
+    for (int i = 0; i < 4; i++) {
+        saved[i] = bufs[i]->p;
+        bufs[i]->p = bufs[i]->n ? base + off[i] : nullptr;
+    }
+    const bool timing = plan->timing;
+    plan->timing = false;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int st = plan->prec == FFTSO3_FP64 ? run<double>(plan, false, d_coef, d_grid, s)
+                                             : run<float>(plan, false, d_coef, d_grid, s);
+    plan->timing = timing;
+    for (int i = 0; i < 4; i++) bufs[i]->p = saved[i];
+    return st;
 }
 
 const char *fftso3_status_string(int status)

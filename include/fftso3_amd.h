@@ -117,6 +117,17 @@ int fftso3_forward(fftso3_plan_t plan, const void *d_grid, void *d_coef, void *s
 /* coefficients -> grid (inverse_transform at every grid point) */
 int fftso3_inverse(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream);
 
+/* fftso3_inverse that allocates nothing and uses none of the plan's workspaces: the spectrum
+ * workspace and coefficient staging live in a static 128 MiB device array of the library (one per
+ * GPU, held until the process ends), so it can be recorded under stream capture or called from
+ * serving loops that must not allocate.  No counterpart in the reference; it is for callers that
+ * cannot allocate.  Plans whose inverse needs more than 128 MiB of workspace (e.g. more than 6
+ * complex functions at l_max = 63 in FP32) are rejected with FFTSO3_EINVAL.  Every call on a GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (other streams, other threads)
+ * race, so the caller orders them.  It always runs eagerly (graph mode and per-stage timing of the
+ * plan do not apply) on the device that was current when the plan was created. */
+int fftso3_inverse_noalloc(fftso3_plan_t plan, const void *d_coef, void *d_grid, void *stream);
+
 /* Pointwise inverse of function fn of the plan's batch at n_points arbitrary points
  * (inverse_transform(F, alpha, beta, gamma), src/fdcl_FFTSO3.cpp:498-531 and 1141-1167;
  * inverse_transform(F, theta, phi), src/fdcl_FFTS2.cpp:194-230 and 500-512), batched:
