@@ -284,6 +284,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 comparison runs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB device to device (PMC calibration)")
     ap.add_argument("--workload", default="so3_b64x256", choices=["so3_b64x256"] + sorted(OTHER_WORKLOADS),
@@ -370,8 +371,36 @@ def main():
         extra.update({"fp64_gpu_value": v64, "speedup_vs_fp64_gpu": value / v64,
                       "inverse_grid_rel_l2_vs_fp64": gerr, "forward_coef_rel_l2_vs_fp64": ferr,
                       "forward_coef_max_abs_vs_fp64": ferr_max})
-        del g64, p64
         log("FP64 GPU path: %.1f transforms/s; errors grid %.2e coef %.2e rt %.2e" % (v64, gerr, ferr, rt_err))
+        if args.precision == "fp32" and not args.no_split:
+            # the split-BF16 precisions of the same batch-wide path, timed and held to FP64 alike
+            split = {}
+            for sp in ("bf16x6", "bf16x3"):
+                ps = fftso3_amd.Plan(B - 1, n_fn, "so3", "complex", sp)
+                gs = torch.empty_like(grid)
+                Fo = torch.empty_like(Fs)
+                ps.inverse(Fs, gs)
+                ps.forward(gs, Fo)
+                torch.cuda.synchronize()
+                ks = max(2, min(5, args.steps))
+                t0 = time.perf_counter()
+                for _ in range(ks):
+                    ps.inverse(Fs, gs)
+                    ps.forward(gs, Fo)
+                torch.cuda.synchronize()
+                vs = n_fn * ks / (time.perf_counter() - t0)
+                rts = float(torch.linalg.norm(Fo.to(torch.complex128) - Fd) / torch.linalg.norm(Fd))
+                ges = float(torch.linalg.norm(gs.to(torch.complex128) - g64) / torch.linalg.norm(g64))
+                ps.forward(g64.to(torch.complex64), Fo)
+                torch.cuda.synchronize()
+                fes = float(torch.linalg.norm(Fo.to(torch.complex128) - F64o) / torch.linalg.norm(F64o))
+                split[sp] = {"value": vs, "speedup_vs_fp64_gpu": vs / v64, "roundtrip_rel_l2": rts,
+                             "inverse_grid_rel_l2_vs_fp64": ges, "forward_coef_rel_l2_vs_fp64": fes,
+                             "bar": PARITY_BAR[sp], "path": ps.describe().strip()}
+                log("%s: %.1f transforms/s; errors grid %.2e coef %.2e rt %.2e" % (sp, vs, ges, fes, rts))
+                del ps, gs, Fo
+            extra["split_bf16"] = split
+        del g64, p64
         # function 0 against the CPU oracle
         from oracle import oracle
         g_ref = oracle.so3_inverse(B, F64[0])

@@ -3,6 +3,7 @@
 #include "mfma.hpp"
 #include "fft128.hpp"
 #include "mfma32.hpp"
+#include "mfma_split.hpp"
 
 #include "../../include/fftso3_amd.h"
 
@@ -339,8 +340,9 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
         const size_t rcb = (size_t)B * sizeof(RecCoef);
         const bool fits = B <= MXK_MAXROWS && mxk_lds_bytes(N, B, np, true) + rcb <= (size_t)kMaxLds &&
                           mxk_lds_bytes(N, B, np, false) + rcb <= (size_t)kMaxLds;
+        const bool split = pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6;
         c->mxk = pl->prec == FFTSO3_FP32 && N > 128 && fits && !pl->force_chunked;
-        c->mx = (pl->prec == FFTSO3_FP32 && N <= 128 && !pl->force_chunked) || c->mxk;
+        c->mx = ((pl->prec == FFTSO3_FP32 || split) && N <= 128 && !pl->force_chunked) || c->mxk;
     }
     c->chunk = c->mx ? (int)c->fns.size() : pl->C;
     // large bandlimits on the chunked kernels: coefficients go through PT[e][fi] so the contraction
@@ -832,7 +834,11 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             StageTimer t(pl, s, 1);
             a.src = pl->d_ws.p;
             a.dst = pl->d_fc.p;
-            if (c->mxk)
+            if (pl->prec == FFTSO3_BF16X6)
+                hipLaunchKernelGGL((contract_fwd_mxs<NP, 3>), dim3(mgrid), dim3(256), mxs_fwd_lds_bytes(3) + rcb, s, a);
+            else if (pl->prec == FFTSO3_BF16X3)
+                hipLaunchKernelGGL((contract_fwd_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_fwd_lds_bytes(2) + rcb, s, a);
+            else if (c->mxk)
                 hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(mgrid), dim3(256),
                                    mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a);
             else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2 && fwd_grouped()) {
@@ -890,7 +896,11 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         StageTimer t(pl, s, 4);
         a.src = pl->d_fc.p;
         a.dst = pl->d_ws.p;
-        if (c->mxk)
+        if (pl->prec == FFTSO3_BF16X6)
+            hipLaunchKernelGGL((contract_inv_mxs<NP, 3>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(3) + rcb, s, a);
+        else if (pl->prec == FFTSO3_BF16X3)
+            hipLaunchKernelGGL((contract_inv_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(2) + rcb, s, a);
+        else if (c->mxk)
         {
             if (mxk_inv8()) {
                 // 8 waves per workgroup over blocks of 256 columns: half as many units
@@ -1201,6 +1211,14 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv_bf16<4, 3>));
     ALLOW_LDS((contract_inv_bf16<2, 2>));
     ALLOW_LDS((contract_inv_bf16<2, 3>));
+    ALLOW_LDS((contract_fwd_mxs<4, 2>));
+    ALLOW_LDS((contract_fwd_mxs<4, 3>));
+    ALLOW_LDS((contract_fwd_mxs<2, 2>));
+    ALLOW_LDS((contract_fwd_mxs<2, 3>));
+    ALLOW_LDS((contract_inv_mxs<4, 2>));
+    ALLOW_LDS((contract_inv_mxs<4, 3>));
+    ALLOW_LDS((contract_inv_mxs<2, 2>));
+    ALLOW_LDS((contract_inv_mxs<2, 3>));
 #undef ALLOW_LDS
     kernel_attrs_done = true;
     return FFTSO3_OK;
@@ -1475,6 +1493,11 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
                     for (int gil : {0, 1, 2})
                         if (klo == 0 || gil == 0) cands.push_back({false, plan->C, klo, sp, gil});
         for (int C : {1, 2, 4, 8}) cands.push_back({true, C, plan->klo, 1, plan->gil});
+    } else {
+        // split-BF16: the batch-wide kernels with each group split over 1, 2 or 4 workgroups, or
+        // the chunked kernels (8 functions per pass)
+        for (int sp : {1, 2, 4}) cands.push_back({false, 8, 0, sp, 0});
+        cands.push_back({true, 8, 0, 1, 0});
     }
     if (cands.empty()) return FFTSO3_OK;
     const bool timing = plan->timing, graph = plan->graph;

@@ -323,3 +323,51 @@ def test_group_interleave_fp32(fftso3, torch_cuda, oracle, monkeypatch, gil, kin
     for i in (0, n - 1):
         ref = oracle.so3_inverse(B, F[i]) if kind == "complex" else oracle.so3_real_inverse(B, F[i])
         assert rel_l2(g[i], ref.reshape(-1)) < FP32_TOL
+
+
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("domain,B,kind", [("so3", 64, "complex"), ("so3", 64, "real"), ("so3", 20, "complex"),
+                                           ("s2", 64, "complex"), ("s2", 45, "real")])
+def test_split_bf16_batchwide(fftso3, torch_cuda, oracle, prec, domain, B, kind):
+    """Split-BF16 on the batch-wide kernels (contract_fwd_mxs / contract_inv_mxs): 37 functions
+    leave a partial last column block; every function round trips, the first and the last
+    against the oracle.  BF16X6 holds the FP32 bar (1e-6 at l_max = 63), BF16X3 its own 5e-5."""
+    torch = torch_cuda
+    n = 37
+    cplx = kind == "complex"
+    if domain == "so3":
+        F = inputs.so3_coefficients(B, n, 7000 + B, complex_=cplx, scale=lambda l: 1.0 / (1.0 + l))
+    else:
+        F = inputs.s2_coefficients(B, n, 7000 + B, complex_=cplx, scale=lambda l: 1.0 / (1.0 + l))
+    plan = fftso3.Plan(B - 1, n, domain, kind, prec)
+    assert "batch-wide" in plan.describe()
+    gdt, cdt = plan.dtypes()
+    g = plan.inverse(_to_dev(torch, F, cdt))
+    c = plan.forward(g)
+    torch.cuda.synchronize()
+    g = g.cpu().numpy().reshape(n, -1)
+    c = c.cpu().numpy().reshape(n, -1)
+    tol = (FP32_TOL_B64 if B == 64 else FP32_TOL) if prec == "bf16x6" else SPLIT_TOL["bf16x3"]
+    assert rel_l2(c, F) < tol
+    for i in (0, n - 1):
+        if domain == "so3":
+            ref = oracle.so3_inverse(B, F[i]) if cplx else oracle.so3_real_inverse(B, F[i])
+        else:
+            ref = oracle.s2_inverse(B, F[i]) if cplx else oracle.s2_real_inverse(B, F[i])
+        assert rel_l2(g[i], ref) < tol, i
+    plan.close()
+
+
+def test_split_bf16_chunked_kernels(fftso3, torch_cuda, oracle):
+    """set_chunk(8) keeps the chunked split-BF16 kernels (mfma.hpp) reachable at l_max <= 63."""
+    torch = torch_cuda
+    B, n = 20, 11
+    F = inputs.so3_coefficients(B, n, 7100)
+    plan = fftso3.Plan(B - 1, n, "so3", "complex", "bf16x6", chunk=8)
+    assert "chunked" in plan.describe()
+    g = plan.inverse(_to_dev(torch, F, torch.complex64))
+    c = plan.forward(g)
+    torch.cuda.synchronize()
+    g = g.cpu().numpy().reshape(n, -1)
+    assert rel_l2(c.cpu().numpy().reshape(n, -1), F) < FP32_TOL
+    assert rel_l2(g[n - 1], oracle.so3_inverse(B, F[n - 1])) < FP32_TOL
