@@ -31,6 +31,8 @@ _SIGS = {
     "fftso3_plan_get_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "fftso3_plan_describe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     "fftso3_plan_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fftso3_rotate_s2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
     "fftso3_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
                                        ctypes.c_void_p, ctypes.c_void_p]),
     "fftso3_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -210,6 +212,24 @@ class Plan:
         _check(lib().fftso3_evaluate(self._h, int(fn), ctypes.c_void_p(coef.data_ptr()),
                                      ctypes.c_void_p(angles.data_ptr()), n, ctypes.c_void_p(out.data_ptr()),
                                      self._stream(stream)))
+        return out
+
+    def rotate_s2(self, F, angles, out=None, stream=None):
+        """S^2 coefficients F ((l_max+1)^2 float64, complex128 for complex plans) rotated by each
+        rotation of angles (float64 [n, 3]: alpha, beta, gamma): G_i^l = U^l(R_i) F^l (real plans)
+        or D^l(R_i) F^l (complex plans).  Needs an SO(3) plan of one bandlimit."""
+        import torch
+        dt = torch.float64 if self.kind == "real" else torch.complex128
+        nc = (self.l_max + 1) ** 2
+        self._validate(F, nc, dt, "coefficients")
+        if angles.dtype != torch.float64 or not angles.is_cuda or not angles.is_contiguous() or angles.numel() % 3:
+            raise ValueError("angles must be a contiguous float64 CUDA tensor with 3 values per rotation")
+        n = angles.numel() // 3
+        if out is None:
+            out = torch.empty(n * nc, dtype=dt, device=F.device)
+        self._validate(out, n * nc, dt, "rotated coefficients")
+        _check(lib().fftso3_rotate_s2(self._h, ctypes.c_void_p(F.data_ptr()), ctypes.c_void_p(angles.data_ptr()), n,
+                                      ctypes.c_void_p(out.data_ptr()), self._stream(stream)))
         return out
 
     def set_graph(self, on=True):

@@ -88,6 +88,9 @@ public:
     // to the device on a stream of the plan's own.
     using Fill = std::function<void(long i0, long i1, double *out)>;
     void forward_fill(int domain, int l_max, int kind, int precision, const Fill &fill, void *coef);
+    // S^2 coefficients F ((l_max+1)^2, double or complex) rotated by n rotations (angles: alpha,
+    // beta, gamma per rotation) into G (n x (l_max+1)^2), through the SO(3) plan's tables
+    void rotate_s2(int l_max, int kind, int precision, const void *F, const double *angles, long n, void *G);
 
 private:
     void prepare(int domain, int l_max, int n_fn, int kind, int precision);
@@ -163,6 +166,10 @@ public:
     // pointwise inverse at many rotations on the GPU: angles holds (alpha, beta, gamma) per point
     std::vector<complex<double>> inverse_transform_points(const FFTSO3_matrix_complex &F,
                                                           const std::vector<double> &angles);
+
+    // S^2 coefficients rotated by many rotations on the GPU: G_i^l = D^l(R_i) F^l (wigner_D,
+    // src/fdcl_FFTSO3.cpp:201-224), FP64; l_max of F is the object's
+    std::vector<FFTS2_matrix_complex> rotate_batch(const FFTS2_matrix_complex &F, const std::vector<Matrix3d> &R);
 
     // character chi_l(theta) = sin((2l+1) theta / 2) / sin(theta / 2) (src/fdcl_FFTSO3.cpp:404-415)
     std::vector<double> character(double theta);
@@ -262,6 +269,8 @@ public:
     // S^2 workflows of example/example3.cpp:50-53 and example/example3_SSM.hpp:62-70:
     // real spherical-harmonic coefficients of a rotated function, G^l = U^l(R) F^l
     FFTS2_matrix_real rotate(const FFTS2_matrix_real &F, const Matrix3d &R);
+    // the same for many rotations at once, on the GPU (FP64); l_max of F is the object's
+    std::vector<FFTS2_matrix_real> rotate_batch(const FFTS2_matrix_real &F, const std::vector<Matrix3d> &R);
     // the matching cost J(R) = sum_l (G^l)^T U^l(R) F^l at every rotation of the SO(3) grid
     // [k][j1][j2] = (alpha_j1, beta_k, gamma_j2), l <= l_max: one real inverse transform on the GPU
     // of H^l_mn = G^l_m F^l_n / (2l+1)

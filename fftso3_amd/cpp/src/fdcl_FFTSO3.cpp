@@ -635,6 +635,42 @@ FFTS2_matrix_real FFTSO3_real::rotate(const FFTS2_matrix_real &F, const Matrix3d
     return G;
 }
 
+static std::vector<double> euler_angles_of(const std::vector<Matrix3d> &R)
+{
+    std::vector<double> ang;
+    ang.reserve(3 * R.size());
+    for (const Matrix3d &r : R) {
+        const std::vector<double> abg = R2Euler323(r);
+        ang.insert(ang.end(), abg.begin(), abg.end());
+    }
+    return ang;
+}
+
+std::vector<FFTS2_matrix_real> FFTSO3_real::rotate_batch(const FFTS2_matrix_real &F, const std::vector<Matrix3d> &R)
+{
+    if (F.l_max != l_max) throw std::runtime_error("rotate_batch: F.l_max must equal the object's l_max");
+    const std::vector<double> ang = euler_angles_of(R);
+    const long nc = (long)(l_max + 1) * (l_max + 1);
+    std::vector<double> G((size_t)nc * R.size());
+    dev_->rotate_s2(l_max, FFTSO3_REAL, precision_, F.data(), ang.data(), (long)R.size(), G.data());
+    std::vector<FFTS2_matrix_real> out(R.size(), FFTS2_matrix_real(l_max));
+    for (size_t i = 0; i < R.size(); i++) std::copy(G.begin() + i * nc, G.begin() + (i + 1) * nc, out[i].v.begin());
+    return out;
+}
+
+std::vector<FFTS2_matrix_complex> FFTSO3_complex::rotate_batch(const FFTS2_matrix_complex &F,
+                                                              const std::vector<Matrix3d> &R)
+{
+    if (F.l_max != l_max) throw std::runtime_error("rotate_batch: F.l_max must equal the object's l_max");
+    const std::vector<double> ang = euler_angles_of(R);
+    const long nc = (long)(l_max + 1) * (l_max + 1);
+    std::vector<complex<double>> G((size_t)nc * R.size());
+    dev_->rotate_s2(l_max, FFTSO3_COMPLEX, precision_, F.data(), ang.data(), (long)R.size(), G.data());
+    std::vector<FFTS2_matrix_complex> out(R.size(), FFTS2_matrix_complex(l_max));
+    for (size_t i = 0; i < R.size(); i++) std::copy(G.begin() + i * nc, G.begin() + (i + 1) * nc, out[i].v.begin());
+    return out;
+}
+
 std::vector<double> FFTSO3_real::correlation(const FFTS2_matrix_real &F, const FFTS2_matrix_real &G)
 {
     const int L = l_max;

@@ -133,6 +133,36 @@ void DevicePlan::forward(int domain, int l_max, int n_fn, int kind, int precisio
     download(d_coef_, coef_elems_ * w, static_cast<double *>(coef), precision, stage_);
 }
 
+void DevicePlan::rotate_s2(int l_max, int kind, int precision, const void *F, const double *angles, long n, void *G)
+{
+    if (n <= 0) return;
+    prepare(FFTSO3_DOMAIN_SO3, l_max, 1, kind, precision);
+    const size_t w = kind == FFTSO3_COMPLEX ? 2 : 1;
+    const size_t nc = (size_t)(l_max + 1) * (l_max + 1) * w;
+    void *dF = nullptr, *dA = nullptr, *dG = nullptr;
+    auto release = [&]() {
+        if (dF) (void)hipFree(dF);
+        if (dA) (void)hipFree(dA);
+        if (dG) (void)hipFree(dG);
+    };
+    try {
+        check_hip(hipMalloc(&dF, nc * sizeof(double)), "hipMalloc(rotation input)");
+        check_hip(hipMalloc(&dA, (size_t)n * 3 * sizeof(double)), "hipMalloc(rotation angles)");
+        check_hip(hipMalloc(&dG, (size_t)n * nc * sizeof(double)), "hipMalloc(rotation output)");
+        check_hip(hipMemcpy(dF, F, nc * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        check_hip(hipMemcpy(dA, angles, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        check_abi(fftso3_rotate_s2(plan_, static_cast<const double *>(dF), static_cast<const double *>(dA), n,
+                                   static_cast<double *>(dG), nullptr),
+                  "fftso3_rotate_s2");
+        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check_hip(hipMemcpy(G, dG, (size_t)n * nc * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+}
+
 void DevicePlan::inverse(int domain, int l_max, int n_fn, int kind, int precision, const void *coef, void *grid)
 {
     prepare(domain, l_max, n_fn, kind, precision);

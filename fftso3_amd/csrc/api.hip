@@ -1211,6 +1211,9 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv_bf16<4, 3>));
     ALLOW_LDS((contract_inv_bf16<2, 2>));
     ALLOW_LDS((contract_inv_bf16<2, 3>));
+    // s2_rotate also has static LDS: its dynamic part is at most the 128 KB accumulator
+    HIP_TRY(hipFuncSetAttribute((const void *)(s2_rotate<true>), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+    HIP_TRY(hipFuncSetAttribute((const void *)(s2_rotate<false>), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
     ALLOW_LDS((contract_fwd_mxs<4, 2>));
     ALLOW_LDS((contract_fwd_mxs<4, 3>));
     ALLOW_LDS((contract_fwd_mxs<2, 2>));
@@ -1464,6 +1467,43 @@ int fftso3_evaluate(fftso3_plan_t plan, int fn, const void *d_coef, const double
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     return plan->prec == FFTSO3_FP64 ? evaluate<double>(plan, fn, d_coef, d_angles, n_points, d_values, s)
                                      : evaluate<float>(plan, fn, d_coef, d_angles, n_points, d_values, s);
+}
+
+int fftso3_rotate_s2(fftso3_plan_t plan, const double *d_F, const double *d_angles, long n_rot, double *d_G,
+                     void *stream)
+{
+    if (int st = check_valid(plan)) return st;
+    if (plan->domain != FFTSO3_DOMAIN_SO3) return fail(FFTSO3_EINVAL, "rotation needs an SO(3) plan (its Wigner tables)");
+    if (plan->classes.size() != 1) return fail(FFTSO3_EINVAL, "rotation needs a plan of one bandlimit");
+    if (n_rot < 0) return fail(FFTSO3_EINVAL, "n_rot must be >= 0 (got %ld)", n_rot);
+    if (n_rot == 0) return FFTSO3_OK;
+    if (!d_F || !d_angles || !d_G) return fail(FFTSO3_EINVAL, "NULL device pointer");
+    Class *c = plan->classes[0];
+    const bool real = plan->kind == FFTSO3_REAL;
+    const int W = real ? 1 : 2, B = c->B;
+    RotArgs a;
+    a.B = B;
+    a.rt = rec_tables(c);
+    a.F = d_F;
+    a.ang = d_angles;
+    a.nrot = n_rot;
+    a.G = d_G;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int grid = (int)std::min<long>(n_rot, 65536);
+    // degree ranges whose coefficients fit the LDS accumulator (one range up to l_max = 127 real)
+    const size_t cap = (size_t)128 * 1024;
+    for (int l0 = 0; l0 < B;) {
+        int l1 = l0 + 1;
+        while (l1 < B && (size_t)((l1 + 1) * (l1 + 1) - l0 * l0) * W * sizeof(double) <= cap) l1++;
+        a.l0 = l0;
+        a.l1 = l1;
+        const size_t lds = (size_t)(l1 * l1 - l0 * l0) * W * sizeof(double);
+        if (real) hipLaunchKernelGGL((s2_rotate<true>), dim3(grid), dim3(256), lds, s, a);
+        else hipLaunchKernelGGL((s2_rotate<false>), dim3(grid), dim3(256), lds, s, a);
+        HIP_TRY(hipGetLastError());
+        l0 = l1;
+    }
+    return FFTSO3_OK;
 }
 
 int fftso3_plan_set_graph(fftso3_plan_t plan, int on)

@@ -1052,3 +1052,148 @@ __global__ void __launch_bounds__(256) eval_points(EvalArgs<T> a)
         __syncthreads();
     }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Rotation of spherical-harmonic coefficients by many rotations (FP64):
+//   complex: G^l_a = sum_b D^l_ab(R) F^l_b, D^l_ab = exp(-i a alpha) d^l_ab(beta) exp(-i b gamma)
+//            (wigner_D, src/fdcl_FFTSO3.cpp:201-224)
+//   real:    G^l = U^l(R) F^l with U^l = conj(T^l) D^l (T^l)^T (real_harmonics,
+//            src/fdcl_FFTSO3.cpp:1062-1105; example/example3.cpp:50-53)
+// One workgroup per rotation.  Threads take the sets (m0, n0), m0 >= n0 >= 0, largest first, and
+// run the recursions of the canonical groups (m0, n0) and (m0, -n0), which give d^l at the 8
+// index pairs (+-m0, +-n0) and (+-n0, +-m0) (the symmetries of common.hpp).  Each quad
+// {(+-a, +-b)} closes under T, so its U (or D) entries are l-independent combinations of its 4 d
+// values, formed once per set; G^l accumulates in LDS with FP64 atomic adds, for the degrees
+// [l0, l1) of the launch (the LDS holds their coefficients).
+// ---------------------------------------------------------------------------------------------
+struct RotArgs {
+    int B;
+    RecTables rt;         // SO(3) tables of the plan's class (cb / lnc / lns replaced per rotation)
+    const double *F;      // B^2 coefficients, real or complex interleaved
+    const double *ang;    // (alpha, beta, gamma) per rotation
+    long nrot;
+    double *G;            // nrot x B^2 coefficients
+    int l0, l1;           // degrees of this launch
+};
+
+// the (up to) 4 index pairs of a quad {(+-a, +-b)}, a, b >= 0: index i over ms = {a, -a}, j over
+// ns = {b, -b}; na = 1 when a = 0 (one row), nb = 1 when b = 0
+struct RotQuad {
+    int a, b, na, nb;
+    double kr[2][2][2][2];   // real: U(ms[i], ns[j]) = sum_{i',j'} kr[i][j][i'][j'] d(ms[i'], ns[j'])
+    double pr[2][2], pi[2][2];  // complex: D(ms[i], ns[j]) = (pr + i pi)[i][j] d(ms[i], ns[j])
+    __device__ __forceinline__ void init(int a_, int b_, double al, double ga)
+    {
+        a = a_; b = b_; na = a ? 2 : 1; nb = b ? 2 : 1;
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) {
+                const int mi = i ? -a : a, nj = j ? -b : b;
+                double s, c;
+                sincos(-((double)mi * al + (double)nj * ga), &s, &c);
+                pr[i][j] = c;
+                pi[i][j] = s;
+            }
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) {
+                const int mi = i ? -a : a, nj = j ? -b : b;
+                for (int i2 = 0; i2 < 2; i2++)
+                    for (int j2 = 0; j2 < 2; j2++) {
+                        const int mc = i2 ? -a : a, ne = j2 ? -b : b;
+                        // conj(T(mi, mc)) T(nj, ne) exp(-i(mc alpha + ne gamma)): real part
+                        const double2 t1 = tmat<double2>(mi, mc), t2 = tmat<double2>(nj, ne);
+                        const double xr = t1.x * t2.x + t1.y * t2.y, xi = t1.x * t2.y - t1.y * t2.x;
+                        kr[i][j][i2][j2] = (i2 < na && j2 < nb) ? xr * pr[i2][j2] - xi * pi[i2][j2] : 0.0;
+                    }
+            }
+    }
+    // accumulate the quad's contributions at degree l: dv[i][j] = d^l(ms[i], ns[j])
+    template <bool REAL>
+    __device__ __forceinline__ void add(double *acc, int e0, int l, const double (&dv)[2][2], const double *F) const
+    {
+        const int base = l * l + l;
+        for (int i = 0; i < na; i++) {
+            const int mi = i ? -a : a;
+            double gr = 0.0, gi = 0.0;
+            for (int j = 0; j < nb; j++) {
+                const int nj = j ? -b : b;
+                if (REAL) {
+                    double u = 0.0;
+                    for (int i2 = 0; i2 < na; i2++)
+                        for (int j2 = 0; j2 < nb; j2++) u += kr[i][j][i2][j2] * dv[i2][j2];
+                    gr += u * F[base + nj];
+                } else {
+                    const double dr = pr[i][j] * dv[i][j], di = pi[i][j] * dv[i][j];
+                    const double fr = F[2 * (base + nj)], fi = F[2 * (base + nj) + 1];
+                    gr += dr * fr - di * fi;
+                    gi += dr * fi + di * fr;
+                }
+            }
+            if (REAL) atomicAdd(acc + (base + mi - e0), gr);
+            else {
+                atomicAdd(acc + 2 * (base + mi - e0), gr);
+                atomicAdd(acc + 2 * (base + mi - e0) + 1, gi);
+            }
+        }
+    }
+};
+
+template <bool REAL>
+__global__ void __launch_bounds__(256) s2_rotate(RotArgs a)
+{
+    extern __shared__ double acc[];   // coefficients l0^2 .. l1^2 - 1 (complex: re, im)
+    __shared__ double sh[3];
+    constexpr int W = REAL ? 1 : 2;
+    const int tid = threadIdx.x, B = a.B;
+    const int e0 = a.l0 * a.l0, ne = W * (a.l1 * a.l1 - e0);
+    const int nsets = a.l1 * (a.l1 + 1) / 2;          // (m0, n0), 0 <= n0 <= m0 < l1
+    for (long r = blockIdx.x; r < a.nrot; r += gridDim.x) {
+        const double tp = 2.0 * M_PI;
+        const double al = a.ang[3 * r], ga = a.ang[3 * r + 2];
+        double be = fmod(a.ang[3 * r + 1], tp);
+        if (be < 0) be += tp;
+        const bool fold = be > M_PI;           // d(2 pi - b) = (-1)^(m-n) d(b)
+        if (fold) be = tp - be;
+        for (int e = tid; e < ne; e += 256) acc[e] = 0.0;
+        if (tid == 0) {
+            sh[0] = cos(be);
+            sh[1] = log(cos(0.5 * be));
+            sh[2] = log(sin(0.5 * be));
+        }
+        __syncthreads();
+        RecTables t = a.rt;
+        t.cb = sh;
+        t.lnc = sh + 1;
+        t.lns = sh + 2;
+        for (int u = tid; u < nsets; u += 256) {
+            int m0 = (int)((sqrt(8.0 * u + 1.0) - 1.0) * 0.5);
+            while (m0 * (m0 + 1) / 2 > u) m0--;
+            while ((m0 + 1) * (m0 + 2) / 2 <= u) m0++;
+            const int n0 = u - m0 * (m0 + 1) / 2;
+            const double s = (double)parity_sign(m0 + n0);
+            const double f = fold ? s : 1.0;    // (-1)^(m-n) for every pair of both quads
+            RotQuad q1, q2;
+            q1.init(m0, n0, al, ga);
+            const bool two = n0 != m0;
+            if (two) q2.init(n0, m0, al, ga);
+            WignerRec rp, rm;
+            rp.init(t, B, m0 * m0 + n0 + m0, m0, n0, 0);
+            if (n0 > 0) rm.init(t, B, m0 * m0 - n0 + m0, m0, -n0, 0);
+            for (int l = m0; l < a.l1; l++) {
+                const double dp = l == m0 ? rp.d1 : rp.step(l);
+                const double dm = n0 > 0 ? (l == m0 ? rm.d1 : rm.step(l)) : dp;
+                if (l < a.l0) continue;
+                // quad 1 over (+-m0) x (+-n0); quad 2 over (+-n0) x (+-m0)
+                const double d1[2][2] = {{f * dp, f * dm}, {f * s * dm, f * s * dp}};
+                q1.add<REAL>(acc, e0, l, d1, a.F);
+                if (two) {
+                    const double d2[2][2] = {{f * s * dp, f * dm}, {f * s * dm, f * dp}};
+                    q2.add<REAL>(acc, e0, l, d2, a.F);
+                }
+            }
+        }
+        __syncthreads();
+        double *G = a.G + r * (long)W * B * B + (long)W * e0;
+        for (int e = tid; e < ne; e += 256) G[e] = acc[e];
+        __syncthreads();
+    }
+}

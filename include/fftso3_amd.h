@@ -139,6 +139,20 @@ int fftso3_inverse_noalloc(fftso3_plan_t plan, const void *d_coef, void *d_grid,
 int fftso3_evaluate(fftso3_plan_t plan, int fn, const void *d_coef, const double *d_angles, long n_points,
                     void *d_values, void *stream);
 
+/* Rotation of spherical-harmonic coefficients by n_rot rotations, through an SO(3) plan of one
+ * bandlimit (its Wigner-d tables; l_max is the plan's):
+ *   FFTSO3_REAL plans:    G_i^l = U^l(R_i) F^l, U the real harmonics (real_harmonics,
+ *                         src/fdcl_FFTSO3.cpp:1062-1105; the rotation of example/example3.cpp:50-53)
+ *   FFTSO3_COMPLEX plans: G_i^l = D^l(R_i) F^l (wigner_D, src/fdcl_FFTSO3.cpp:201-224)
+ *   d_F       (l_max+1)^2 coefficients in the S^2 packed layout l^2 + m + l, double (complex
+ *             double interleaved for complex plans), whatever the plan's precision
+ *   d_angles  n_rot x (alpha, beta, gamma), the 3-2-3 Euler angles of R_i
+ *   d_G       n_rot x (l_max+1)^2 coefficients, same type as d_F
+ * FP64 recursion and sums (accumulated with atomic adds, so the last bits may differ between
+ * calls). */
+int fftso3_rotate_s2(fftso3_plan_t plan, const double *d_F, const double *d_angles, long n_rot, double *d_G,
+                     void *stream);
+
 /* --- diagnostics ----------------------------------------------------------------------------- */
 
 const char *fftso3_status_string(int status);

@@ -324,6 +324,37 @@ static int gpu_checks()
         const size_t best = std::max_element(J.begin(), J.end()) - J.begin();
         expect("s2_correlation_peak_at_true_rotation", (double)(best != ((size_t)k0 * N + a0) * N + g0), 0);
     }
+    // batched GPU rotation of S^2 coefficients equals the host rotation G = U(R) F (real) and
+    // D(R) F computed from wigner_D (complex)
+    {
+        const int L = 9;
+        fdcl::FFTSO3_real fr(L);
+        fdcl::FFTS2_matrix_real F(L);
+        F.setRandom();
+        std::vector<fdcl::Matrix3d> Rs;
+        for (int i = 0; i < 7; i++) Rs.push_back(fdcl::Euler3232R(0.3 + 0.7 * i, 0.2 + 0.4 * i, -1.0 + 0.5 * i));
+        const auto Gb = fr.rotate_batch(F, Rs);
+        double e = 0;
+        for (size_t i = 0; i < Rs.size(); i++) e = std::max(e, rel(Gb[i].v, fr.rotate(F, Rs[i]).v));
+        expect("s2_rotate_batch_real_vs_host", e, 1e-12);
+        fdcl::FFTSO3_complex fc(L);
+        fdcl::FFTS2_matrix_complex Fc(L);
+        Fc.setRandom();
+        const auto Gc = fc.rotate_batch(Fc, Rs);
+        double ec = 0;
+        for (size_t i = 0; i < Rs.size(); i++) {
+            const auto D = fc.wigner_D(Rs[i]);
+            std::vector<fdcl::complex<double>> ref(Fc.v.size());
+            for (int l = 0; l <= L; l++)
+                for (int m = -l; m <= l; m++) {
+                    fdcl::complex<double> s = 0;
+                    for (int n = -l; n <= l; n++) s += D(l, m, n) * Fc(l, n);
+                    ref[(size_t)l * l + m + l] = s;
+                }
+            ec = std::max(ec, rel(Gc[i].v, ref));
+        }
+        expect("s2_rotate_batch_complex_vs_wigner_D", ec, 1e-12);
+    }
     // forward_transform(std::function): the chunked, pipelined host sampling must give exactly the
     // coefficients of the grid overload on the same samples (l_max = 40: several 4 MB chunks)
     {
