@@ -190,7 +190,8 @@ def bench_other(args, name):
     dev = torch.device("cuda:0")
     if name == "so3_uneven512_real":
         Bs, coefs = inputs.workload_uneven(512)
-        plan = fftso3_amd.Plan([B - 1 for B in Bs], domain="so3", kind="real", precision=args.precision)
+        plan = fftso3_amd.Plan([B - 1 for B in Bs], domain="so3", kind="real", precision=args.precision,
+                               flags=fftso3_amd.FLAG_PER_CLASS_LAUNCHES if args.per_class_launches else 0)
         F = torch.from_numpy(np.concatenate(coefs)).to(torch.float32).to(dev)
         domain, real = "so3", True
         ref_fn = lambda i: (oracle.so3_real_forward(Bs[i], oracle.so3_real_inverse(Bs[i], coefs[i], 0), 0))
@@ -259,7 +260,8 @@ def bench_other(args, name):
     out = {"metric": "%s fwd+inv transforms/s (%s)" % (name, args.precision), "value": value, "unit": "transforms/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": OTHER_WORKLOADS[name], "functions": n_fn, "precision": args.precision},
+           "config": {"workload": OTHER_WORKLOADS[name], "functions": n_fn, "precision": args.precision,
+                      "schedule": plan.describe().strip().splitlines()[0]},
            "roofline": {"bound": "hbm", "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": bl,
                         "avg_launch_ms": avg_ms},
@@ -285,6 +287,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-BF16 comparison runs")
+    ap.add_argument("--per-class-launches", action="store_true",
+                    help="uneven workload: one launch per bandlimit class and stage instead of one per stage")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB device to device (PMC calibration)")
     ap.add_argument("--workload", default="so3_b64x256", choices=["so3_b64x256"] + sorted(OTHER_WORKLOADS),

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libfftso3_amd.so")
 DOMAIN = {"so3": 0, "s2": 1}
 KIND = {"complex": 0, "real": 1}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
+FLAG_PER_CLASS_LAUNCHES = 1
 STAGES = ("fft_fwd", "contract_fwd", "basis_fwd", "basis_inv", "contract_inv", "fft_inv")
 
 _lib = None
@@ -99,19 +100,19 @@ class Plan:
     forward(grid) -> coefficients, inverse(coefficients) -> grid, on torch CUDA tensors laid out
     as include/fftso3_amd.h describes (batch packed back to back; complex interleaved)."""
 
-    def __init__(self, l_max, n_fn=1, domain="so3", kind="complex", precision="fp64", chunk=None):
+    def __init__(self, l_max, n_fn=1, domain="so3", kind="complex", precision="fp64", chunk=None, flags=0):
         L = lib()
         self.domain, self.kind, self.precision = domain, kind, precision
         h = ctypes.c_void_p()
         if isinstance(l_max, (list, tuple)):
             arr = (ctypes.c_int * len(l_max))(*l_max)
             _check(L.fftso3_plan_create_uneven(ctypes.byref(h), DOMAIN[domain], arr, len(l_max), KIND[kind],
-                                               PRECISION[precision], 0))
+                                               PRECISION[precision], int(flags)))
             self.l_max = list(l_max)
             self.n_fn = len(l_max)
         else:
             _check(L.fftso3_plan_create(ctypes.byref(h), DOMAIN[domain], int(l_max), int(n_fn), KIND[kind],
-                                        PRECISION[precision], 0))
+                                        PRECISION[precision], int(flags)))
             self.l_max = int(l_max)
             self.n_fn = int(n_fn)
         self._h = h

@@ -4,6 +4,7 @@
 #include "fft128.hpp"
 #include "mfma32.hpp"
 #include "mfma_split.hpp"
+#include "multi.hpp"
 
 #include "../../include/fftso3_amd.h"
 
@@ -129,6 +130,8 @@ struct Class {
     int row_tile = 0, col_tile = 0; // two-pass fallback tiles (S^2 rings: divide N)
     int so3_tile = 0;               // SO(3) row / column passes (partial last block)
     bool mx = false;                // batch-wide FP32-MFMA contraction (whole class in one pass)
+    int r16 = 0;                    // R of the N = 16R slice FFTs (fft_r16.hpp) of this class, or 0
+    size_t ws_off = 0, fc_off = 0, tmp_off = 0;   // the class's region of the plan's buffers (unified plans)
     bool mxk = false;               // ... with the K-chunked kernels (128 < N, B <= 128)
     bool pt = false;                // chunked contraction reads / writes transposed coefficients PT
     DevBuf d_dtab, d_dtab_off;      // precomputed d table of the chunked path at large B (or empty)
@@ -161,6 +164,14 @@ struct fftso3_plan_s {
     int klo = 0;                       // row blocking of the batch-wide workspace (ws_row)
     int gil = 1;                       // group interleave of the batch-wide workspace (ws_rec)
     int split_mul = 1;                 // multiplier of the automatic group split (batch-wide kernels)
+    // unified schedule: every bandlimit class in one launch per stage (multi.hpp), each class in
+    // its own region of the workspaces; the queues of (entry, unit) items per stage
+    bool multi_on = true;              // allowed (autotune may switch it off)
+    bool unified = false;              // in use for the current schedule
+    DevBuf d_q_fft, d_q_cf, d_q_ci, d_q_tr;
+    int nq_fft = 0, nq_cf = 0, nq_ci = 0, nq_tr = 0;
+    std::vector<Class *> uorder;                       // classes, largest bandlimit first
+    std::vector<std::pair<Class *, int>> ent_cf, ent_ci;   // contraction entries (class, kind)
     // graph mode: one instantiated graph per (direction, input, output, stream)
     bool graph = false;
     struct GraphEntry {
@@ -458,6 +469,9 @@ static int ws_klo()
 }
 
 
+int build_unified(fftso3_plan_s *pl);
+bool unified_eligible(const fftso3_plan_s *pl);
+
 int finalize_plan(fftso3_plan_s *pl)
 {
     pl->drop_graphs();  // captured graphs hold the old buffers
@@ -483,16 +497,34 @@ int finalize_plan(fftso3_plan_s *pl)
     for (auto *c : pl->classes) {
         if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
         if ((st = c->d_fn.upload(c->fns))) return st;
+        // N = 16R slice FFTs (fft_r16.hpp) for the FP32 batch-wide SO(3) classes of B = 8R, R <= 6
+        const int R = c->B / 8;
+        c->r16 = (pl->prec == FFTSO3_FP32 && !s2 && c->mx && !c->mxk && c->B % 8 == 0 &&
+                  (R == 1 || R == 2 || R == 3 || R == 4 || R == 6)) ? R : 0;
+    }
+    pl->unified = unified_eligible(pl);
+    for (auto *c : pl->classes) {
         const size_t B = c->B, N = c->N, C = c->chunk;
         const int klo = pl->klo;
-        size_t w = (size_t)ws_rows(c->mx, klo, s2 ? (int)B : (int)(B * B), (int)N) * C * (s2 ? 2 : 4) * pl->esz;
+        const size_t w = (size_t)ws_rows(c->mx, klo, s2 ? (int)B : (int)(B * B), (int)N) * C * (s2 ? 2 : 4) * pl->esz;
+        const size_t cf = C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz;
+        if (pl->unified) {
+            // every class in a region of its own, 256-byte aligned
+            c->ws_off = ws;
+            c->fc_off = fc;
+            c->tmp_off = tmpb;
+            ws += (w + 255) & ~(size_t)255;
+            fc += (cf + 255) & ~(size_t)255;
+            if (pl->kind == FFTSO3_REAL) tmpb += (cf + 255) & ~(size_t)255;
+            continue;
+        }
+        c->ws_off = c->fc_off = c->tmp_off = 0;
         ws = std::max(ws, w);
         if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
-        if (pl->kind == FFTSO3_REAL)
-            fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
+        if (pl->kind == FFTSO3_REAL) fc = std::max(fc, cf);
         if (c->mx || c->pt) {
-            fc = std::max(fc, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);  // PT
-            if (pl->kind == FFTSO3_REAL) tmpb = std::max(tmpb, C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz);
+            fc = std::max(fc, cf);  // PT
+            if (pl->kind == FFTSO3_REAL) tmpb = std::max(tmpb, cf);
         }
     }
     // ws and PT are read up to 512 bytes (ws with 4 groups interleaved: 2 KB) past a row start
@@ -502,6 +534,7 @@ int finalize_plan(fftso3_plan_s *pl)
     if ((st = pl->d_dummy.resize(256)) || (st = pl->d_ws.resize(ws)) || (st = pl->d_scratch.resize(scr)) ||
         (st = pl->d_fc.resize(fc)) || (st = pl->d_tmp.resize(tmpb)))
         return st;
+    if (pl->unified && (st = build_unified(pl))) return st;
     pl->valid = true;
     return FFTSO3_OK;
 }
@@ -956,6 +989,32 @@ int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const v
     }
 }
 
+// the N = 16R slice FFTs of one class (fft_r16.hpp)
+template <int R>
+int launch_r16_R(bool fwd, bool real, const FftArgs<float> &a, hipStream_t s)
+{
+    const int grid = xcd_grid(r16_units<R>(a.ca.nfc));
+    const size_t lds = r16_lds_bytes<R>();
+    if (fwd && real) hipLaunchKernelGGL((so3_fft_r16<R, true, true>), dim3(grid), dim3(1024), lds, s, a);
+    else if (fwd) hipLaunchKernelGGL((so3_fft_r16<R, true, false>), dim3(grid), dim3(1024), lds, s, a);
+    else if (real) hipLaunchKernelGGL((so3_fft_r16<R, false, true>), dim3(grid), dim3(1024), lds, s, a);
+    else hipLaunchKernelGGL((so3_fft_r16<R, false, false>), dim3(grid), dim3(1024), lds, s, a);
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
+int launch_r16(int R, bool fwd, bool real, const FftArgs<float> &a, hipStream_t s)
+{
+    switch (R) {
+    case 1: return launch_r16_R<1>(fwd, real, a, s);
+    case 2: return launch_r16_R<2>(fwd, real, a, s);
+    case 3: return launch_r16_R<3>(fwd, real, a, s);
+    case 4: return launch_r16_R<4>(fwd, real, a, s);
+    case 6: return launch_r16_R<6>(fwd, real, a, s);
+    default: return fail(FFTSO3_EINVAL, "no 16R slice FFT for R = %d", R);
+    }
+}
+
 template <typename T>
 int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void *in, void *out, hipStream_t s)
 {
@@ -999,6 +1058,10 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                         else hipLaunchKernelGGL((so3_fft128_fwd2<false>), dim3(g2), dim3(1024), lds, s, a);
                     } else if (real) hipLaunchKernelGGL((so3_fft128_fwd<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                     else hipLaunchKernelGGL((so3_fft128_fwd<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                }
+            } else if (c->r16) {
+                if constexpr (sizeof(T) == 4) {
+                    if ((st = launch_r16(c->r16, true, real, a, s))) return st;
                 }
             } else if (c->fused) {
                 size_t lds = (size_t)N * N * sizeof(T2) + tables;
@@ -1089,6 +1152,10 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
                     } else if (real) hipLaunchKernelGGL((so3_fft128_inv<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                     else hipLaunchKernelGGL((so3_fft128_inv<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
                 }
+            } else if (c->r16) {
+                if constexpr (sizeof(T) == 4) {
+                    if ((st = launch_r16(c->r16, false, real, a, s))) return st;
+                }
             } else if (c->fused) {
                 size_t lds = (size_t)N * N * sizeof(T2) + tables;
                 if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
@@ -1111,9 +1178,290 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
     return FFTSO3_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// unified schedule (multi.hpp): every class of an uneven FP32 SO(3) plan in one launch per stage
+// ---------------------------------------------------------------------------------------------
+bool unified_eligible(const fftso3_plan_s *pl)
+{
+    if (!pl->multi_on || pl->prec != FFTSO3_FP32 || pl->domain != FFTSO3_DOMAIN_SO3 || pl->force_chunked ||
+        pl->klo != 0 || pl->classes.size() < 2 || pl->classes.size() > MULTI_MAX_FFT)
+        return false;
+    for (const Class *c : pl->classes) {
+        if (!c->mx || c->mxk) return false;
+        if (c->N == 128) {
+            if (ws_gil(pl, c) != 1 || !fft128_pairs() || !fft128_pairs_inv() || !fwd_grouped() || !fwd_mx16() ||
+                !inv_grouped())
+                return false;
+        } else if (!c->r16) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// contraction arguments of a class as its own launch would use them; kind as in multi.hpp
+// (forward: 0 one group per workgroup, 2 / 3 the grouped N = 128 kernels for groups with more /
+// at most 32 degrees; inverse: 0 one group, 1 grouped)
+MxArgs make_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, int kind)
+{
+    MxArgs a;
+    a.ca = ca;
+    a.rt = rec_tables(c);
+    a.items = (const int *)c->d_items_mx.p;
+    a.nitems = c->nitems_mx;
+    a.gpw = 1;
+    a.g0 = 0;
+    a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);
+    a.src = a.dst = nullptr;
+    const int nfb = (ca.nfc * 4 * 2 + 127) / 128;
+    a.nsplit = std::max(1, std::min(nfb, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
+    if (kind == 0) return a;
+    a.gpw = 2;
+    a.nitems = c->nitems_mx / 2;
+    const int nfbg = (ca.nfc + 7) / 8;
+    a.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
+    if (!fwd) return a;
+    const int gsplit = c->B > 32 ? (c->B - 32) * (c->B - 32) : 0;
+    if (kind == 2) {
+        a.nitems = gsplit / 2;
+    } else {
+        a.g0 = gsplit;
+        a.nitems = c->nitems_mx / 2 - gsplit / 2;
+    }
+    return a;
+}
+
+// per-XCD queues, interleaved so that position p belongs to XCD p % 8
+struct XcdQueue {
+    std::vector<int2> x[XCD_COUNT];
+    void add_units(int entry, int n)   // the units of an xcd_unit launch
+    {
+        const int q = (n + XCD_COUNT - 1) / XCD_COUNT;
+        for (int u = 0; u < n; u++) x[u / q].push_back(make_int2(entry, u));
+    }
+    void add_dealt(int entry, int n, int run)   // the units of an xcd_deal launch
+    {
+        const int grid = deal_grid(n, run);
+        for (int b = 0; b < grid; b++) {
+            const int xx = b % XCD_COUNT, i = b / XCD_COUNT;
+            const int u = ((i / run) * XCD_COUNT + xx) * run + i % run;
+            if (u < n) x[xx].push_back(make_int2(entry, u));
+        }
+    }
+    std::vector<int2> flat() const
+    {
+        size_t L = 0;
+        for (const auto &v : x) L = std::max(L, v.size());
+        std::vector<int2> out(L * XCD_COUNT, make_int2(-1, 0));
+        for (size_t p = 0; p < out.size(); p++)
+            if (p / XCD_COUNT < x[p % XCD_COUNT].size()) out[p] = x[p % XCD_COUNT][p / XCD_COUNT];
+        return out;
+    }
+};
+
+int fft_units(const Class *c)
+{
+    if (c->N == 128) return so3_fft128_fwd2_units((int)c->fns.size());
+    switch (c->r16) {
+    case 1: return r16_units<1>((int)c->fns.size());
+    case 2: return r16_units<2>((int)c->fns.size());
+    case 3: return r16_units<3>((int)c->fns.size());
+    case 4: return r16_units<4>((int)c->fns.size());
+    default: return r16_units<6>((int)c->fns.size());
+    }
+}
+
+size_t fft_lds(const Class *c)
+{
+    if (c->N == 128) return (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
+    switch (c->r16) {
+    case 1: return r16_lds_bytes<1>();
+    case 2: return r16_lds_bytes<2>();
+    case 3: return r16_lds_bytes<3>();
+    case 4: return r16_lds_bytes<4>();
+    default: return r16_lds_bytes<6>();
+    }
+}
+
+int build_unified(fftso3_plan_s *pl)
+{
+    pl->uorder = pl->classes;
+    std::stable_sort(pl->uorder.begin(), pl->uorder.end(), [](const Class *a, const Class *b) { return a->N > b->N; });
+    pl->ent_cf.clear();
+    pl->ent_ci.clear();
+    XcdQueue qf, qcf, qci, qtr;
+    for (int i = 0; i < (int)pl->uorder.size(); i++) {
+        Class *c = pl->uorder[i];
+        const int nfc = (int)c->fns.size();
+        const ChunkArgs ca = chunk_args<float>(pl, c, 0, nfc);
+        qf.add_units(i, fft_units(c));
+        qtr.add_units(i, coef_transpose_units<float2>(so3_ncoef(c->B), nfc));
+        const std::vector<int> kf = c->N == 128 ? std::vector<int>{2, 3} : std::vector<int>{0};
+        for (int kind : kf) {
+            const MxArgs a = make_mx(pl, c, ca, true, kind);
+            if (a.nitems <= 0) continue;
+            qcf.add_dealt((int)pl->ent_cf.size(), a.nitems * a.nsplit, a.gpw > 1 ? a.nsplit : a.nsplit << ca.gil);
+            pl->ent_cf.push_back({c, kind});
+        }
+        const MxArgs ai = make_mx(pl, c, ca, false, c->N == 128 ? 1 : 0);
+        qci.add_dealt((int)pl->ent_ci.size(), ai.nitems * ai.nsplit, ai.gpw > 1 ? ai.nsplit : ai.nsplit << ca.gil);
+        pl->ent_ci.push_back({c, c->N == 128 ? 1 : 0});
+    }
+    if (pl->ent_cf.size() > MULTI_MAX_MX || pl->ent_ci.size() > MULTI_MAX_MX)
+        return fail(FFTSO3_EUNSUPPORTED, "too many contraction entries for one launch");
+    int st;
+    const std::vector<int2> f = qf.flat(), cf = qcf.flat(), ci = qci.flat(), tr = qtr.flat();
+    if ((st = pl->d_q_fft.upload(f)) || (st = pl->d_q_cf.upload(cf)) || (st = pl->d_q_ci.upload(ci)) ||
+        (st = pl->d_q_tr.upload(tr)))
+        return st;
+    pl->nq_fft = (int)f.size();
+    pl->nq_cf = (int)cf.size();
+    pl->nq_ci = (int)ci.size();
+    pl->nq_tr = (int)tr.size();
+    return FFTSO3_OK;
+}
+
+int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStream_t s)
+{
+    const bool real = pl->kind == FFTSO3_REAL;
+    const int ncls = (int)pl->uorder.size();
+    char *ws = static_cast<char *>(pl->d_ws.p), *fc = static_cast<char *>(pl->d_fc.p);
+    char *tmp = static_cast<char *>(pl->d_tmp.p);
+    // per-class chunk arguments; the real path stages packed complex coefficients per class-local
+    // function in the class's d_tmp region
+    std::vector<ChunkArgs> ca(ncls), ct(ncls);
+    for (int i = 0; i < ncls; i++) {
+        Class *c = pl->uorder[i];
+        ca[i] = chunk_args<float>(pl, c, 0, (int)c->fns.size());
+        ct[i] = ca[i];
+        if (real) {
+            ct[i].fn = reinterpret_cast<const int *>(c->d_ident.p);
+            ct[i].coef_off = reinterpret_cast<const long *>(c->d_ws_off.p);
+        }
+    }
+    auto fft_stage = [&](bool f, const void *src, void *dst) -> int {
+        MultiFft m;
+        std::memset(&m, 0, sizeof(m));
+        size_t lds = 0;
+        for (int i = 0; i < ncls; i++) {
+            Class *c = pl->uorder[i];
+            m.c[i] = fft_args<float>(pl, c, ca[i], f ? src : ws + c->ws_off, f ? (void *)(ws + c->ws_off) : dst);
+            m.kind[i] = c->N == 128 ? 8 : c->r16;
+            lds = std::max(lds, fft_lds(c));
+        }
+        m.items = reinterpret_cast<const int2 *>(pl->d_q_fft.p);
+        m.nitems = pl->nq_fft;
+        if (f && real) hipLaunchKernelGGL((so3_fft_multi<true, true>), dim3(persistent_grid(so3_fft_multi<true, true>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
+        else if (f) hipLaunchKernelGGL((so3_fft_multi<true, false>), dim3(persistent_grid(so3_fft_multi<true, false>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
+        else if (real) hipLaunchKernelGGL((so3_fft_multi<false, true>), dim3(persistent_grid(so3_fft_multi<false, true>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
+        else hipLaunchKernelGGL((so3_fft_multi<false, false>), dim3(persistent_grid(so3_fft_multi<false, false>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
+        HIP_TRY(hipGetLastError());
+        return FFTSO3_OK;
+    };
+    auto contract_stage = [&](bool f) -> int {
+        MultiMx m;
+        std::memset(&m, 0, sizeof(m));
+        const auto &ent = f ? pl->ent_cf : pl->ent_ci;
+        size_t lds = 0;
+        for (size_t e = 0; e < ent.size(); e++) {
+            Class *c = ent[e].first;
+            const int i = (int)(std::find(pl->uorder.begin(), pl->uorder.end(), c) - pl->uorder.begin());
+            MxArgs a = make_mx(pl, c, ca[i], f, ent[e].second);
+            a.src = f ? (const void *)(ws + c->ws_off) : (const void *)(fc + c->fc_off);
+            a.dst = f ? (void *)(fc + c->fc_off) : (void *)(ws + c->ws_off);
+            m.c[e] = a;
+            m.kind[e] = ent[e].second;
+            const size_t rcb = (size_t)c->B * sizeof(RecCoef);
+            const size_t l = ent[e].second == 0 ? MX_LDS_BYTES + rcb
+                           : f ? (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb
+                               : (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
+            lds = std::max(lds, l);
+        }
+        m.items = reinterpret_cast<const int2 *>((f ? pl->d_q_cf : pl->d_q_ci).p);
+        m.nitems = f ? pl->nq_cf : pl->nq_ci;
+        if (f) hipLaunchKernelGGL(contract_fwd_multi, dim3(persistent_grid(contract_fwd_multi, lds, m.nitems, 256)), dim3(256), lds, s, m);
+        else hipLaunchKernelGGL(contract_inv_multi, dim3(persistent_grid(contract_inv_multi, lds, m.nitems, 256)), dim3(256), lds, s, m);
+        HIP_TRY(hipGetLastError());
+        return FFTSO3_OK;
+    };
+    auto transpose_stage = [&](int dir, const void *src_user, void *dst_user) -> int {
+        MultiTr m;
+        std::memset(&m, 0, sizeof(m));
+        for (int i = 0; i < ncls; i++) {
+            Class *c = pl->uorder[i];
+            m.ca[i] = ct[i];
+            m.ncoef[i] = (int)so3_ncoef(c->B);
+            if (dir == 0) {
+                m.src[i] = fc + c->fc_off;
+                m.dst[i] = real ? (void *)(tmp + c->tmp_off) : dst_user;
+            } else {
+                m.src[i] = real ? (const void *)(tmp + c->tmp_off) : src_user;
+                m.dst[i] = fc + c->fc_off;
+            }
+        }
+        m.items = reinterpret_cast<const int2 *>(pl->d_q_tr.p);
+        m.nitems = pl->nq_tr;
+        if (dir == 0) hipLaunchKernelGGL((coef_transpose_multi<0>), dim3(persistent_grid(coef_transpose_multi<0>, 0, m.nitems, 256)), dim3(256), 0, s, m);
+        else hipLaunchKernelGGL((coef_transpose_multi<1>), dim3(persistent_grid(coef_transpose_multi<1>, 0, m.nitems, 256)), dim3(256), 0, s, m);
+        HIP_TRY(hipGetLastError());
+        return FFTSO3_OK;
+    };
+    auto tmix_stage = [&](int dir, const void *src_user, void *dst_user) -> int {
+        MultiTmix m;
+        std::memset(&m, 0, sizeof(m));
+        long tot = 0;
+        for (int i = 0; i < ncls; i++) {
+            Class *c = pl->uorder[i];
+            m.ca[i] = ca[i];
+            m.src[i] = dir == 0 ? (const void *)(tmp + c->tmp_off) : src_user;
+            m.dst[i] = dir == 0 ? dst_user : (void *)(tmp + c->tmp_off);
+            m.ws_stride[i] = so3_ncoef(c->B);
+            m.start[i] = tot;
+            tot += so3_tmix_orbits(c->B) * (long)c->fns.size();
+        }
+        m.start[ncls] = tot;
+        m.ncls = ncls;
+        const int blocks = (int)std::min<long>(8192, (tot + 255) / 256);
+        if (dir == 0) hipLaunchKernelGGL((so3_tmix_multi<0>), dim3(blocks), dim3(256), 0, s, m);
+        else hipLaunchKernelGGL((so3_tmix_multi<1>), dim3(blocks), dim3(256), 0, s, m);
+        HIP_TRY(hipGetLastError());
+        return FFTSO3_OK;
+    };
+    int st;
+    if (fwd) {
+        {
+            StageTimer t(pl, s, 0);
+            if ((st = fft_stage(true, in, nullptr))) return st;
+        }
+        {
+            StageTimer t(pl, s, 1);
+            if ((st = contract_stage(true))) return st;
+        }
+        StageTimer t(pl, s, 2);
+        if ((st = transpose_stage(0, nullptr, out))) return st;
+        if (real && (st = tmix_stage(0, nullptr, out))) return st;
+    } else {
+        {
+            StageTimer t(pl, s, 3);
+            if (real && (st = tmix_stage(1, in, nullptr))) return st;
+            if ((st = transpose_stage(1, in, nullptr))) return st;
+        }
+        {
+            StageTimer t(pl, s, 4);
+            if ((st = contract_stage(false))) return st;
+        }
+        StageTimer t(pl, s, 5);
+        if ((st = fft_stage(false, nullptr, out))) return st;
+    }
+    return FFTSO3_OK;
+}
+
 template <typename T>
 int run(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (pl->unified) return run_unified(pl, fwd, in, out, s);
+    }
     int st;
     for (auto *c : pl->classes) {
         const int n = (int)c->fns.size();
@@ -1214,6 +1562,23 @@ int set_kernel_attrs()
     // s2_rotate also has static LDS: its dynamic part is at most the 128 KB accumulator
     HIP_TRY(hipFuncSetAttribute((const void *)(s2_rotate<true>), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
     HIP_TRY(hipFuncSetAttribute((const void *)(s2_rotate<false>), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
+#define ALLOW_R16(R)                                 \
+    ALLOW_LDS((so3_fft_r16<R, true, true>));         \
+    ALLOW_LDS((so3_fft_r16<R, true, false>));        \
+    ALLOW_LDS((so3_fft_r16<R, false, true>));        \
+    ALLOW_LDS((so3_fft_r16<R, false, false>))
+    ALLOW_R16(1);
+    ALLOW_R16(2);
+    ALLOW_R16(3);
+    ALLOW_R16(4);
+    ALLOW_R16(6);
+#undef ALLOW_R16
+    ALLOW_LDS((so3_fft_multi<true, true>));
+    ALLOW_LDS((so3_fft_multi<true, false>));
+    ALLOW_LDS((so3_fft_multi<false, true>));
+    ALLOW_LDS((so3_fft_multi<false, false>));
+    ALLOW_LDS(contract_fwd_multi);
+    ALLOW_LDS(contract_inv_multi);
     ALLOW_LDS((contract_fwd_mxs<4, 2>));
     ALLOW_LDS((contract_fwd_mxs<4, 3>));
     ALLOW_LDS((contract_fwd_mxs<2, 2>));
@@ -1227,8 +1592,9 @@ int set_kernel_attrs()
     return FFTSO3_OK;
 }
 
-int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int kind, int precision)
+int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int kind, int precision, unsigned flags)
 {
+    if (flags & ~(unsigned)FFTSO3_FLAG_PER_CLASS_LAUNCHES) return fail(FFTSO3_EINVAL, "unknown plan flags 0x%x", flags);
     if (!out) return fail(FFTSO3_EINVAL, "plan pointer is NULL");
     *out = nullptr;
     if (domain != FFTSO3_DOMAIN_SO3 && domain != FFTSO3_DOMAIN_S2) return fail(FFTSO3_EINVAL, "unknown domain %d", domain);
@@ -1247,6 +1613,7 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
     fftso3_plan_s *pl = new (std::nothrow) fftso3_plan_s();
     if (!pl) return fail(FFTSO3_ENOMEM, "host allocation failed");
     pl->device = dev;
+    pl->multi_on = !(flags & FFTSO3_FLAG_PER_CLASS_LAUNCHES);
     pl->domain = domain;
     pl->kind = kind;
     pl->prec = precision;
@@ -1294,18 +1661,16 @@ extern "C" {
 
 int fftso3_plan_create(fftso3_plan_t *plan, int domain, int l_max, int n_fn, int kind, int precision, unsigned flags)
 {
-    (void)flags;
     if (n_fn <= 0) return fail(FFTSO3_EINVAL, "n_fn must be positive (got %d)", n_fn);
     std::vector<int> lm((size_t)n_fn, l_max);
-    return create_common(plan, domain, lm.data(), n_fn, kind, precision);
+    return create_common(plan, domain, lm.data(), n_fn, kind, precision, flags);
 }
 
 int fftso3_plan_create_uneven(fftso3_plan_t *plan, int domain, const int *l_max, int n_fn, int kind, int precision,
                               unsigned flags)
 {
-    (void)flags;
     if (!l_max) return fail(FFTSO3_EINVAL, "l_max array is NULL");
-    return create_common(plan, domain, l_max, n_fn, kind, precision);
+    return create_common(plan, domain, l_max, n_fn, kind, precision, flags);
 }
 
 int fftso3_plan_destroy(fftso3_plan_t plan)
@@ -1338,9 +1703,10 @@ int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len)
         const char *path = c->mxk ? "batch-wide mfma (k-chunked)" : c->mx ? "batch-wide mfma" : "chunked";
         char line[256];
         ChunkArgs ca = chunk_args<float>(plan, const_cast<Class *>(c), 0, (int)c->fns.size());
-        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s\n", c->B - 1,
+        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s\n", c->B - 1,
                       (int)c->fns.size(), prec, path, c->chunk, c->mx ? plan->klo : 0, ca.gil, plan->split_mul,
-                      c->d_dtab.p ? " d-table" : "", plan->graph ? " graph" : "");
+                      c->r16 ? " fft16R" : "", plan->unified ? " unified" : "", c->d_dtab.p ? " d-table" : "",
+                      plan->graph ? " graph" : "");
         out += line;
     }
     std::snprintf(buf, (size_t)len, "%s", out.c_str());
@@ -1523,7 +1889,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     if (!d_grid || !d_coef) return fail(FFTSO3_EINVAL, "NULL device pointer");
     if (reps < 1 || reps > 100) return fail(FFTSO3_EINVAL, "reps must be in [1, 100] (got %d)", reps);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    struct Cand { bool chunked; int C, klo, split, gil; };
+    struct Cand { bool chunked; int C, klo, split, gil; bool multi = true; };
     std::vector<Cand> cands;
     const bool bf16 = plan->prec == FFTSO3_BF16X3 || plan->prec == FFTSO3_BF16X6;
     if (!bf16) {
@@ -1532,6 +1898,9 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
                 for (int sp : {1, 2, 4})
                     for (int gil : {0, 1, 2})
                         if (klo == 0 || gil == 0) cands.push_back({false, plan->C, klo, sp, gil});
+        // several bandlimit classes: also one launch per class and stage instead of the unified launches
+        if (plan->prec == FFTSO3_FP32 && plan->classes.size() > 1)
+            for (int sp : {1, 2}) cands.push_back({false, plan->C, 0, sp, 1, false});
         for (int C : {1, 2, 4, 8}) cands.push_back({true, C, plan->klo, 1, plan->gil});
     } else {
         // split-BF16: the batch-wide kernels with each group split over 1, 2 or 4 workgroups, or
@@ -1555,6 +1924,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         plan->klo = c.klo;
         plan->split_mul = c.split;
         plan->gil = c.gil;
+        plan->multi_on = c.multi;
         if ((st = finalize_plan(plan))) break;
         if ((st = run_dispatch(plan, false, d_coef, d_grid, s)) || (st = run_dispatch(plan, true, d_grid, d_coef, s))) break;
         (void)hipEventRecord(e0, s);
@@ -1576,6 +1946,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     plan->klo = win.klo;
     plan->split_mul = win.split;
     plan->gil = win.gil;
+    plan->multi_on = win.multi;
     const int st2 = finalize_plan(plan);
     plan->timing = timing;
     plan->graph = graph;

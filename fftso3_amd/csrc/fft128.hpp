@@ -424,19 +424,21 @@ __global__ void __launch_bounds__(256) s2_fft256_inv(FftArgs<float> a)
 // registers until the second's are ready, and writes each line whole.
 // ---------------------------------------------------------------------------------------------
 
-template <bool REAL>
-__global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
+template <bool REAL, bool OPAQUE = false>
+__device__ __forceinline__ void so3_fft128_fwd2_body(const FftArgs<float> &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *buf = reinterpret_cast<float2 *>(smem_raw);
     float2 *twl = buf + F128 * P128;
     const ChunkArgs &ca = a.ca;
     const int nfp = (ca.nfc + 1) / 2, nunit = nfp * F128;
-    const int unit = xcd_unit(nunit);
-    if (unit >= nunit) return;
+    if (unit < 0 || unit >= nunit) return;
     const int k = unit / nfp, fi = 2 * (unit - k * nfp);
     const bool two = fi + 1 < ca.nfc;
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    // in a persistent loop over units (OPAQUE) the per-lane index arithmetic stays inside the
+    // iteration instead of being hoisted and held in registers across it
+    if (OPAQUE) asm volatile("" : "+v"(tid));
     if (tid < F128) twl[tid] = a.tw[tid];
     float2 vA[2][8], vB[2][8];
     fft128_load<REAL>(a, k, fi, tid, vA);
@@ -475,6 +477,15 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
         S4[o + 4] = h0;
         S4[o + 5] = h1;
     }
+}
+
+// units of so3_fft128_fwd2: (beta index, pair of functions)
+__host__ __device__ inline int so3_fft128_fwd2_units(int nfc) { return (nfc + 1) / 2 * F128; }
+
+template <bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
+{
+    so3_fft128_fwd2_body<REAL>(a, xcd_unit(so3_fft128_fwd2_units(a.ca.nfc)));
 }
 
 // Inverse with the same pairing: both slices' records are gathered at once (whole lines), then
@@ -518,19 +529,21 @@ __device__ __forceinline__ void fft128_inv_slice(const FftArgs<float> &a, float2
     }
 }
 
-template <bool REAL>
-__global__ void __launch_bounds__(1024) so3_fft128_inv2(FftArgs<float> a)
+template <bool REAL, bool OPAQUE = false>
+__device__ __forceinline__ void so3_fft128_inv2_body(const FftArgs<float> &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *buf = reinterpret_cast<float2 *>(smem_raw);
     float2 *twl = buf + F128 * P128;
     const ChunkArgs &ca = a.ca;
     const int nfp = (ca.nfc + 1) / 2, nunit = nfp * F128;
-    const int unit = xcd_unit(nunit);
-    if (unit >= nunit) return;
+    if (unit < 0 || unit >= nunit) return;
     const int k = unit / nfp, fi = 2 * (unit - k * nfp);
     const bool two = fi + 1 < ca.nfc;
-    const int tid = threadIdx.x;
+    int tid = threadIdx.x;
+    // in a persistent loop over units (OPAQUE) the per-lane index arithmetic stays inside the
+    // iteration instead of being hoisted and held in registers across it
+    if (OPAQUE) asm volatile("" : "+v"(tid));
     if (tid < F128) twl[tid] = a.tw[tid];
     float4 q0[4][2], q1[4][2];
     const float4 *G4 = reinterpret_cast<const float4 *>(a.src);
@@ -546,4 +559,13 @@ __global__ void __launch_bounds__(1024) so3_fft128_inv2(FftArgs<float> a)
     if (!two) return;
     __syncthreads();  // the first slice's LDS reads before the second slice's scatter
     fft128_inv_slice<REAL>(a, buf, twl, q1, k, fi + 1, tid);
+}
+
+// units of so3_fft128_inv2: (beta index, pair of functions)
+__host__ __device__ inline int so3_fft128_inv2_units(int nfc) { return (nfc + 1) / 2 * F128; }
+
+template <bool REAL>
+__global__ void __launch_bounds__(1024) so3_fft128_inv2(FftArgs<float> a)
+{
+    so3_fft128_inv2_body<REAL>(a, xcd_unit(so3_fft128_inv2_units(a.ca.nfc)));
 }

@@ -844,17 +844,20 @@ __global__ void __launch_bounds__(32 * C) contract_inv(ConArgs<T> a)
 // ---------------------------------------------------------------------------------------------
 // SO(3) forward epilogue: Fr(l,m,n) = Re sum_{a=+-m, b=+-n} T(m,a) Fc(a,b) conj T(n,b)
 //   (src/fdcl_FFTSO3.cpp:1169-1181).   Inverse prologue: Fc(a,b) = sum_{m=+-a,n=+-b} conj T(m,a) Fr(m,n) T(n,b).
+// orbits {(+-a, +-b)} of every degree of a bandlimit, per function
+__host__ __device__ inline long so3_tmix_orbits(int B) { return (long)B * (B + 1) * (2 * B + 1) / 6; }
+
+// element e (an orbit of one function) of so3_tmix
 template <typename T, int DIR>
-__global__ void so3_tmix(ChunkArgs ca, const void *src, void *dst, long ws_stride)
+__device__ __forceinline__ void so3_tmix_elem(const ChunkArgs &ca, const void *src, void *dst, long ws_stride, long e)
 {
     typedef typename cplx_t<T>::type T2;
     // one thread per orbit {(+-a, +-b)} of a degree l (a, b in [0, l]): its (up to) 4 inputs are
     // read once and its (up to) 4 outputs written; orbits of l start at l (l+1) (2l+1) / 6
     const int B = ca.B;
-    const long no = (long)B * (B + 1) * (2 * B + 1) / 6;
-    const long total = no * ca.nfc;
+    const long no = so3_tmix_orbits(B);
     auto orb_off = [](int l) { return l * (l + 1) * (2 * l + 1) / 6; };
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    {
         const int fi = (int)(e / no);
         const int idx = (int)(e - (long)fi * no);
         int l = (int)cbrtf((float)idx * 3.0f);
@@ -910,6 +913,14 @@ __global__ void so3_tmix(ChunkArgs ca, const void *src, void *dst, long ws_strid
             }
         }
     }
+}
+
+template <typename T, int DIR>
+__global__ void so3_tmix(ChunkArgs ca, const void *src, void *dst, long ws_stride)
+{
+    const long total = so3_tmix_orbits(ca.B) * ca.nfc;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+        so3_tmix_elem<T, DIR>(ca, src, dst, ws_stride, e);
 }
 
 // S^2: forward Fr(l,a) = Re sum_{c=+-a} conj T(a,c) Fc(l,c) (src/fdcl_FFTS2.cpp:424-436);

@@ -45,13 +45,18 @@ struct MxArgs {
 // Work unit of a batch-wide kernel: group a.items[u / nsplit] and its column blocks [fb0, fb1)
 // (nfb blocks of 128 spectrum columns in all).  False for units past the grid or with no blocks;
 // either is uniform over the workgroup.
-__device__ __forceinline__ bool mx_unit(const MxArgs &a, int nfb, int &g, int &fb0, int &fb1)
+// The unit a workgroup of a batch-wide kernel launch takes: groups come largest first, dealt
+// round-robin over the XCDs (xcd_deal) in runs of the nsplit units of a group times the 2^gil
+// groups whose workspace records share lines.
+__device__ __forceinline__ int mx_block_unit(const MxArgs &a)
+{
+    return xcd_deal(a.gpw > 1 ? a.nsplit : a.nsplit << a.ca.gil);
+}
+
+__device__ __forceinline__ bool mx_unit(const MxArgs &a, int nfb, int unit, int &g, int &fb0, int &fb1)
 {
     const int n = a.nitems * a.nsplit;
-    // groups come largest first: deal them round-robin over the XCDs (xcd_deal), in runs of the
-    // nsplit units of a group times the 2^gil groups whose workspace records share lines
-    const int unit = xcd_deal(a.gpw > 1 ? a.nsplit : a.nsplit << a.ca.gil);
-    if (unit >= n) return false;
+    if (unit < 0 || unit >= n) return false;
     const int it = unit / a.nsplit, part = unit - it * a.nsplit;
     g = a.gpw > 1 ? a.g0 + it * a.gpw : a.items[it];
     fb0 = (int)((long)part * nfb / a.nsplit);
@@ -104,7 +109,7 @@ __device__ __forceinline__ int mx_lcol(int c)
 // FULL: N == 128; the lane's operands of D (64 per row tile) then stay in registers for the whole
 // batch and the multiply is unrolled over all 64 steps, with two independent accumulators.
 template <int NP, bool FULL>
-__global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
+__device__ __forceinline__ void contract_fwd_mx32_body(const MxArgs &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float *D = reinterpret_cast<float *>(smem_raw);     // [64][MX_DPITCH], rows l - m0
@@ -113,7 +118,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nrt = (M + 31) / 32;
@@ -252,13 +257,19 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
     }
 }
 
+template <int NP, bool FULL>
+__global__ void __launch_bounds__(256) contract_fwd_mx32(MxArgs a)
+{
+    contract_fwd_mx32_body<NP, FULL>(a, mx_block_unit(a));
+}
+
 // Forward, N = 128, SO(3), workspace with 2^gil groups interleaved (GPW = 2^gil): one workgroup
 // takes the GPW groups whose records share each workspace line, so a block (16 / GPW functions of
 // all GPW groups) is again 512 contiguous bytes of every beta row.  Wave w works on group
 // w / (4 / GPW) and on GPW of its pairs, 32 columns as before; each wave keeps its own group's
 // D in registers, staged two groups at a time through the LDS that later holds the blocks.
 template <int GPW>
-__global__ void __launch_bounds__(256) contract_fwd_mx32g(MxArgs a)
+__device__ __forceinline__ void contract_fwd_mx32g_body(const MxArgs &a, int unit)
 {
     constexpr int NP = 4, FB = 16 / GPW, WPG = 4 / GPW;  // functions per block, waves per group
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -269,7 +280,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32g(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc + FB - 1) / FB;
     int gb, fb0, fb1;
-    if (!mx_unit(a, nfb, gb, fb0, fb1)) return;         // gb: first group of the interleaved set
+    if (!mx_unit(a, nfb, unit, gb, fb0, fb1)) return;         // gb: first group of the interleaved set
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     const int gg = wave / WPG, pg = wave % WPG;          // this wave's group and pair set
@@ -383,6 +394,12 @@ __global__ void __launch_bounds__(256) contract_fwd_mx32g(MxArgs a)
     }
 }
 
+template <int GPW>
+__global__ void __launch_bounds__(256) contract_fwd_mx32g(MxArgs a)
+{
+    contract_fwd_mx32g_body<GPW>(a, mx_block_unit(a));
+}
+
 typedef float f32x4m_t __attribute__((ext_vector_type(4)));
 
 // Forward as contract_fwd_mx32g<2>, on 16 x 16 x 4 MFMA tiles: a group's degree rows are padded to
@@ -391,7 +408,7 @@ typedef float f32x4m_t __attribute__((ext_vector_type(4)));
 // = spectrum(k, column), result row 4(i/16) + j, column i%16.  Wave w takes group w / 2 and pairs
 // 2(w % 2), 2(w % 2) + 1: one 16-column tile per pair (8 functions, re/im).
 template <int NRT>
-__global__ void __launch_bounds__(256, NRT == 2 ? 2 : 1) contract_fwd_mx16g(MxArgs a)
+__device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int unit)
 {
     constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -402,7 +419,7 @@ __global__ void __launch_bounds__(256, NRT == 2 ? 2 : 1) contract_fwd_mx16g(MxAr
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc + FB - 1) / FB;
     int gb, fb0, fb1;
-    if (!mx_unit(a, nfb, gb, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, gb, fb0, fb1)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kq = lane >> 4, cx = lane & 15;
     const int gg = wave / WPG, pg = wave % WPG;
@@ -517,10 +534,16 @@ __global__ void __launch_bounds__(256, NRT == 2 ? 2 : 1) contract_fwd_mx16g(MxAr
     }
 }
 
+template <int NRT>
+__global__ void __launch_bounds__(256, NRT == 2 ? 2 : 1) contract_fwd_mx16g(MxArgs a)
+{
+    contract_fwd_mx16g_body<NRT>(a, mx_block_unit(a));
+}
+
 // Inverse: G_k(fn, pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) PT(packed(l, pair); fn) (c_l = 2l+1
 // for SO(3), 1 for S^2).
 template <int NP>
-__global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
+__device__ __forceinline__ void contract_inv_mx32_body(const MxArgs &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float *D = reinterpret_cast<float *>(smem_raw);     // [64][MX_DPITCH], rows l - m0, columns k
@@ -529,7 +552,7 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, M4 = (M + 3) & ~3, nkt = (N + 31) / 32;
@@ -647,13 +670,19 @@ __global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
     }
 }
 
+template <int NP>
+__global__ void __launch_bounds__(256) contract_inv_mx32(MxArgs a)
+{
+    contract_inv_mx32_body<NP>(a, mx_block_unit(a));
+}
+
 // Inverse, N = 128, SO(3), workspace with two groups interleaved (GPW = 2): one workgroup takes
 // the two groups whose records share each workspace line, so a block (8 functions of both groups)
 // is one contiguous 512-byte run of every beta row and each 128-byte line is written whole by one
 // store instruction.  Waves 0-1 work on the first group, 2-3 on the second, two pairs each; the
 // two groups' c_l d^l blocks sit side by side in LDS.
 template <int GPW>
-__global__ void __launch_bounds__(256) contract_inv_mx32g(MxArgs a)
+__device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int unit)
 {
     static_assert(GPW == 2, "the two groups of one interleaved workspace line");
     constexpr int NP = 4, FB = 16 / GPW, WPG = 4 / GPW;  // functions per block, waves per group
@@ -665,7 +694,7 @@ __global__ void __launch_bounds__(256) contract_inv_mx32g(MxArgs a)
     const int B = ca.B, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc + FB - 1) / FB;
     int gb, fb0, fb1;
-    if (!mx_unit(a, nfb, gb, fb0, fb1)) return;         // gb: first (even) group of the set
+    if (!mx_unit(a, nfb, unit, gb, fb0, fb1)) return;         // gb: first (even) group of the set
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     const int gw = wave / WPG;                           // this wave's group within the set
@@ -765,6 +794,12 @@ __global__ void __launch_bounds__(256) contract_inv_mx32g(MxArgs a)
     }
 }
 
+template <int GPW>
+__global__ void __launch_bounds__(256) contract_inv_mx32g(MxArgs a)
+{
+    contract_inv_mx32g_body<GPW>(a, mx_block_unit(a));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Coefficient transpose between the reference's packed layout (FFTSO3_matrix / FFTS2_matrix
 // storage, one function after another: [function][e]) and PT[e][function].  Tiles of 64 packed
@@ -781,15 +816,14 @@ template <typename T2> inline int coef_transpose_grid(long ncoef, int nfc)
 }
 
 template <int DIR, typename T2 = float2>
-__global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, const void *src, void *dst)
+__device__ __forceinline__ void coef_transpose_body(const ChunkArgs &ca, int ncoef, const void *src, void *dst, int unit)
 {
     constexpr int TE = coef_transpose_te<T2>(), TF = 32;  // tile: TE packed indices x 32 functions
     constexpr int RUNS = TE / 64, NJ = 8 * RUNS;          // 64-index runs per row; items per thread
     __shared__ T2 tile[TF][TE + 1];
     __shared__ long poff[TF];
     const int nte = (ncoef + TE - 1) / TE, ntf = (ca.nfc + TF - 1) / TF, nunit = nte * ntf;
-    const int unit = xcd_unit(nunit);
-    if (unit >= nunit) return;
+    if (unit < 0 || unit >= nunit) return;
     const int et = unit / ntf, ft = unit - et * ntf;
     const int e0 = et * TE, f0 = ft * TF;
     const int t = threadIdx.x;
@@ -840,6 +874,19 @@ __global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, c
     }
 }
 
+// tiles of coef_transpose for ncoef packed indices x nfc functions
+template <typename T2> __host__ __device__ inline int coef_transpose_units(long ncoef, int nfc)
+{
+    constexpr int TE = coef_transpose_te<T2>();
+    return (int)(((ncoef + TE - 1) / TE) * ((nfc + 31) / 32));
+}
+
+template <int DIR, typename T2 = float2>
+__global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, const void *src, void *dst)
+{
+    coef_transpose_body<DIR, T2>(ca, ncoef, src, dst, xcd_unit(coef_transpose_units<T2>(ncoef, ca.nfc)));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Batch-wide contraction for 128 < N with the group's Wigner-d block resident in LDS
 // (32 * ceil(M/32) rows x N columns, pitch N+1: up to M = 128, N = 256 -- S^2 at l_max = 127,
@@ -862,14 +909,14 @@ __host__ __device__ inline size_t mxk_lds_bytes(int N, int B, int NP, bool fwd)
 }
 
 template <int NP>
-__global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
+__device__ __forceinline__ void contract_fwd_mxk_body(const MxArgs &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int N32 = (N + 31) & ~31, NK = N32 / 32;
@@ -963,6 +1010,12 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
     }
 }
 
+template <int NP>
+__global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
+{
+    contract_fwd_mxk_body<NP>(a, mx_block_unit(a));
+}
+
 #undef MXK_LOAD_CHUNK
 
 // Inverse with the group's c_l d^l block resident in LDS ([M4][N+1], rows l - m0 zero-padded to a
@@ -970,14 +1023,14 @@ __global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
 // NW waves per workgroup, each owning 32 columns of a block: 8 waves (2 per SIMD) share the
 // group's d block so one wave's MFMAs cover another's LDS and HBM waits.
 template <int NP, int NW>
-__global__ void __launch_bounds__(64 * NW) contract_inv_mxk(MxArgs a)
+__device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
     const int nfb = (NCOL + 32 * NW - 1) / (32 * NW);       // column blocks of 32 NW
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, M4 = (M + 3) & ~3, DP = N + 1, nkt = (N + 31) / 32;
@@ -1045,6 +1098,12 @@ __global__ void __launch_bounds__(64 * NW) contract_inv_mxk(MxArgs a)
             }
         }
     }
+}
+
+template <int NP, int NW>
+__global__ void __launch_bounds__(64 * NW) contract_inv_mxk(MxArgs a)
+{
+    contract_inv_mxk_body<NP, NW>(a, mx_block_unit(a));
 }
 
 // ---------------------------------------------------------------------------------------------

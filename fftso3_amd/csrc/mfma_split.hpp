@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mxs(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, mx_block_unit(a), g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nrt = (M + 31) / 32, nt = (N + 15) / 16;
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(256) contract_inv_mxs(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, mx_block_unit(a), g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nu = (M + 15) / 16, nkt = (N + 31) / 32;

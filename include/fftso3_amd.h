@@ -67,11 +67,18 @@ enum fftso3_precision {
 
 /* --- plans ----------------------------------------------------------------------------------- */
 
-/* Uniform batch: n_fn functions of bandlimit B = l_max + 1.  flags: reserved, pass 0. */
+/* Plan flags.  By default an FP32 SO(3) plan of several bandlimits (each a multiple of 8, up to
+ * l_max = 63) runs each pipeline stage as one persistent launch over all its bandlimit classes,
+ * draining a static per-XCD queue of (class, unit) items, largest bandlimit first;
+ * FFTSO3_FLAG_PER_CLASS_LAUNCHES asks for one launch per class and stage instead (same results). */
+#define FFTSO3_FLAG_PER_CLASS_LAUNCHES 1u
+
+/* Uniform batch: n_fn functions of bandlimit B = l_max + 1.  flags: 0 or the flags above. */
 int fftso3_plan_create(fftso3_plan_t *plan, int domain, int l_max, int n_fn, int kind, int precision,
                        unsigned flags);
 
-/* Uneven batch: function i has bandlimit l_max[i] + 1. */
+/* Uneven batch: function i has bandlimit l_max[i] + 1 (the reference's objects each have one
+ * bandlimit, src/fdcl_FFTSO3.cpp:8-14; a batch of them of several bandlimits is one plan here). */
 int fftso3_plan_create_uneven(fftso3_plan_t *plan, int domain, const int *l_max, int n_fn, int kind,
                               int precision, unsigned flags);
 

@@ -38,6 +38,7 @@ def test_invalid_arguments_return_einval():
     assert L.fftso3_plan_create(ctypes.byref(h), 0, 5, 1, 3, 0, 0) == 1          # bad kind
     assert L.fftso3_plan_create(ctypes.byref(h), 0, 5, 1, 0, 9, 0) == 1          # bad precision
     assert L.fftso3_plan_create(None, 0, 5, 1, 0, 0, 0) == 1                     # NULL out pointer
+    assert L.fftso3_plan_create(ctypes.byref(h), 0, 5, 1, 0, 0, 6) == 1          # unknown flags
     assert b"precision" in L.fftso3_last_error() or len(L.fftso3_last_error()) > 0
     assert L.fftso3_forward(None, None, None, None) == 1
     assert L.fftso3_plan_destroy(None) == 1
