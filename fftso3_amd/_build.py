@@ -78,3 +78,41 @@ def build_all(verbose=False):
             f.endswith(".cpp") for f in os.listdir(os.path.join(HERE, "cpp", "src"))):
         build_fdcl(verbose)
         build_fdcl_test(verbose)
+
+
+# ---------------------------------------------------------------------------------------------
+# sanitizer builds of the host code (CPU only; clang from the ROCm toolchain, no device code)
+# ---------------------------------------------------------------------------------------------
+CLANGXX = os.path.join(ROCM, "llvm", "bin", "clang++")
+CLANG = os.path.join(ROCM, "llvm", "bin", "clang")
+SAN_DIR = os.path.join(ROOT, "tests", "cpp")
+SANITIZERS = {"asan": ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"],
+              "tsan": ["-fsanitize=thread"]}
+
+
+def build_sanitized(kind):
+    """tests/cpp/test_sanitize_<kind>: the oracle and the class layer's sampling step under the
+    sanitizer (asan: AddressSanitizer + UndefinedBehaviorSanitizer; tsan: ThreadSanitizer)."""
+    flags = SANITIZERS[kind] + ["-g", "-O1", "-fopenmp"]
+    out = os.path.join(SAN_DIR, "test_sanitize_" + kind)
+    obj = out + "_oracle.o"
+    _run([CLANG, "-std=gnu11", "-c", "-o", obj, os.path.join(ROOT, "oracle", "oracle.c")] + flags)
+    _run([CLANGXX, "-std=c++17", "-o", out, os.path.join(SAN_DIR, "test_sanitize.cpp"),
+          os.path.join(HERE, "cpp", "src", "fdcl_sampling.cpp"), obj, "-I", os.path.join(HERE, "cpp", "include"),
+          "-lm", "-Wl,-rpath," + os.path.join(ROCM, "llvm", "lib"), "-L", os.path.join(ROCM, "llvm", "lib")] + flags)
+    return out
+
+
+def build_fdcl_asan():
+    """tests/cpp/test_fdcl_asan: the class layer's host checks (test_fdcl cpu) under
+    AddressSanitizer + UndefinedBehaviorSanitizer."""
+    src_dir = os.path.join(HERE, "cpp", "src")
+    srcs = sorted(os.path.join(src_dir, f) for f in os.listdir(src_dir) if f.endswith(".cpp"))
+    out = os.path.join(SAN_DIR, "test_fdcl_asan")
+    _run([CLANGXX, "-std=c++17", "-g", "-O1", "-fopenmp", "-D__HIP_PLATFORM_AMD__"] + SANITIZERS["asan"] +
+         ["-I", os.path.join(HERE, "cpp", "include"), "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(ROCM, "include"), "-o", out, os.path.join(SAN_DIR, "test_fdcl.cpp")] + srcs +
+         ["-L", HERE, "-lfftso3_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+          "-L", os.path.join(ROCM, "llvm", "lib"),
+          "-Wl,-rpath," + HERE, "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-Wl,-rpath," + os.path.join(ROCM, "llvm", "lib")])
+    return out

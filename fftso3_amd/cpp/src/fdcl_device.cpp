@@ -9,6 +9,7 @@
 #include <string>
 
 #include "fdcl_FFTSO3.hpp"
+#include "fdcl_sampling.hpp"
 
 namespace fdcl
 {
@@ -210,23 +211,9 @@ void DevicePlan::forward_fill(int domain, int l_max, int kind, int precision, co
     for (long c0 = 0; c0 < nv; c0 += chunk, it++) {
         const int b = (int)(it & 1);
         if (it >= 2) check_hip(hipEventSynchronize(static_cast<hipEvent_t>(ev_[b])), "hipEventSynchronize");
-        const long n = std::min(chunk, nv - c0), np = (n + piece - 1) / piece;
+        const long n = std::min(chunk, nv - c0);
         void *buf = pin_[b];
-#pragma omp parallel
-        {
-            std::vector<double> tmp(fp64 ? 0 : (size_t)piece * w);
-#pragma omp for schedule(dynamic, 4)
-            for (long p = 0; p < np; p++) {
-                const long i0 = c0 + p * piece, i1 = std::min(c0 + n, i0 + piece);
-                if (fp64) {
-                    fill(i0, i1, static_cast<double *>(buf) + (i0 - c0) * w);
-                } else {
-                    fill(i0, i1, tmp.data());
-                    float *f = static_cast<float *>(buf) + (i0 - c0) * w;
-                    for (long i = 0; i < (i1 - i0) * w; i++) f[i] = (float)tmp[i];
-                }
-            }
-        }
+        sample_chunk(fill, c0, n, piece, w, fp64, buf);
         check_hip(hipMemcpyAsync(dg + (size_t)c0 * w * rsz, buf, (size_t)n * w * rsz, hipMemcpyHostToDevice, st),
                   "hipMemcpyAsync H2D");
         check_hip(hipEventRecord(static_cast<hipEvent_t>(ev_[b]), st), "hipEventRecord");
