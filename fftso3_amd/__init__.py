@@ -31,6 +31,7 @@ _SIGS = {
     "fftso3_plan_set_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_plan_get_chunk": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "fftso3_plan_describe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "fftso3_plan_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "fftso3_plan_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "fftso3_rotate_s2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
                                         ctypes.c_void_p, ctypes.c_void_p]),
@@ -236,6 +237,10 @@ class Plan:
     def set_graph(self, on=True):
         """Capture each (direction, input, output, stream) into a hipGraph and replay it."""
         _check(lib().fftso3_plan_set_graph(self._h, 1 if on else 0))
+
+    def set_schedule(self, klo=0, gil=1, split=1):
+        """Workspace row blocking, group interleave and group split of the batch-wide kernels."""
+        _check(lib().fftso3_plan_set_schedule(self._h, int(klo), int(gil), int(split)))
 
     def describe(self):
         """The current schedule, one line per bandlimit class."""

@@ -457,18 +457,6 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     c->so3_tile = rt3;
     return FFTSO3_OK;
 }
-// log2 of the run of beta rows a group keeps contiguous in the batch-wide workspace
-static int ws_klo()
-{
-    static const int v = [] {
-        const char *e = getenv("FFTSO3_WS_KLO");
-        const int x = e ? atoi(e) : 0;
-        return x < 0 ? 0 : (x > 7 ? 7 : x);
-    }();
-    return v;
-}
-
-
 int build_unified(fftso3_plan_s *pl);
 bool unified_eligible(const fftso3_plan_s *pl);
 
@@ -583,38 +571,9 @@ int persistent_grid(K kernel, size_t lds, int nunits, int threads = kFftThreads)
     return std::max(XCD_COUNT, (grid / XCD_COUNT) * XCD_COUNT);
 }
 
-bool g_disable_fft128 = false;
-
-template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128 && !g_disable_fft128; }
-// N = 128 forward FFT that takes the two functions sharing each workspace line (gil = 1;
-// FFTSO3_FFT128_PAIRS=0 selects one slice per workgroup)
-static bool fft128_pairs()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FFT128_PAIRS");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
-static bool fft128_pairs_inv()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FFT128_PAIRS_INV");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
-// register-based ring FFTs of the S^2 grid at l_max = 127 (FFTSO3_S2FFT256=0 selects the generic kernels)
-template <typename T> bool use_s2fft256(const Class *c)
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_S2FFT256");
-        return e ? atoi(e) : 1;
-    }();
-    return sizeof(T) == 4 && c->N == 256 && on != 0;
-}
+template <typename T> bool use_fft128(const Class *c) { return sizeof(T) == 4 && c->N == 128; }
+// register-based ring FFTs of the S^2 grid at l_max = 127
+template <typename T> bool use_s2fft256(const Class *c) { return sizeof(T) == 4 && c->N == 256; }
 
 // Group interleave of the workspace (ws_rec) for the N = 128 FFT kernels with the batch-wide FP32
 // MFMA contraction and unblocked rows; 0 everywhere else.  Two groups side by side (gil = 1) let
@@ -624,7 +583,7 @@ template <typename T> bool use_s2fft256(const Class *c)
 static int ws_gil(const fftso3_plan_s *pl, const Class *c)
 {
     const bool ok = pl->prec == FFTSO3_FP32 && c->mx && !c->mxk && c->N == 128 && pl->klo == 0 &&
-                    pl->domain == FFTSO3_DOMAIN_SO3 && !g_disable_fft128;
+                    pl->domain == FFTSO3_DOMAIN_SO3;
     return ok ? pl->gil : 0;
 }
 
@@ -663,26 +622,6 @@ FftArgs<T> fft_args(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, const void
 }
 
 
-// table-fed MFMA forward contraction above l_max = 127 (FFTSO3_FWD_TAB=0 selects the vector kernel)
-// half-staged table forward (FFTSO3_FWD_TAB2=0: the whole spectrum block in LDS at once)
-static bool fwd_tab2()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FWD_TAB2");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
-static bool fwd_tab()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FWD_TAB");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
 template <typename T, int C>
 int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst,
                     hipStream_t s)
@@ -697,13 +636,12 @@ int launch_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, 
     a.src = src;
     a.dst = dst;
     const int np = a.s2 ? 2 : 4, ncol = 2 * np * C;
-    if (fwd && sizeof(T) == 4 && c->d_dtab.p && ncol <= TABF_COLS && (size_t)c->N * TABF_COLS * 4 <= (size_t)kMaxLds &&
-        fwd_tab()) {
+    if (fwd && sizeof(T) == 4 && c->d_dtab.p && ncol <= TABF_COLS && (size_t)c->N * TABF_COLS * 4 <= (size_t)kMaxLds) {
         // FP32 with the d table: MFMA over 32-degree tiles, one workgroup per group
         ConArgs<float> af;
         std::memcpy(&af, &a, sizeof(af));
         const int ngrp = a.s2 ? c->B : c->B * c->B;
-        if (c->B <= 32 * 4 * TABF2_TPW && c->B % 8 == 0 && fwd_tab2()) {
+        if (c->B <= 32 * 4 * TABF2_TPW && c->B % 8 == 0) {
             // the spectrum block staged half a beta range at a time: two workgroups per CU
             const size_t lds = (size_t)(c->N / 2) * TABF_COLS * sizeof(float);
             if (a.s2) hipLaunchKernelGGL((contract_fwd_tab2<2>), dim3(deal_grid(ngrp, 1)), dim3(256), lds, s, af);
@@ -775,58 +713,12 @@ int launch_contract_bf16(Class *c, const ConArgs<float> &a0, bool fwd, hipStream
     return FFTSO3_OK;
 }
 
-// grouped forward contraction on the interleaved workspace (FFTSO3_FWD_GROUPED=0: one group per workgroup)
-static bool fwd_grouped()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FWD_GROUPED");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
-// grouped forward contraction on 16 x 16 x 4 MFMA tiles (FFTSO3_FWD_MX16=0: 32 x 32 x 2 tiles)
-static bool fwd_mx16()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FWD_MX16");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
-// inverse mxk contraction with 8 waves per workgroup (FFTSO3_MXK_INV8=0: 4 waves)
-static bool mxk_inv8()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_MXK_INV8");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
-// inverse contraction over the two groups of each interleaved workspace line in one workgroup
-// (FFTSO3_INV_GROUPED=0 selects one group per workgroup)
-static bool inv_grouped()
-{
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_INV_GROUPED");
-        return e ? atoi(e) : 1;
-    }();
-    return on != 0;
-}
-
 // Threads of a whole-slice FFT workgroup: N^2 / 16 rounded up to whole waves (16 slice elements
 // per thread, the direct DFT's register bound FFT_EPT), so the small slices of low bandlimits do
 // not tie up 1024-thread workgroups with most threads idle and more of them share a CU (measured:
-// 16 elements per thread beat 8, which beat a fixed 1024).  FFTSO3_FUSED_THREADS=0 keeps 1024.
+// 16 elements per thread beat 8, which beat a fixed 1024).
 static int fused_threads(int N)
 {
-    static const int on = [] {
-        const char *e = getenv("FFTSO3_FUSED_THREADS");
-        return e ? atoi(e) : 1;
-    }();
-    if (!on) return kFftThreads;
     const int t = (N * N / FFT_EPT + 63) / 64 * 64;
     return std::max(64, std::min(kFftThreads, t));
 }
@@ -874,7 +766,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             else if (c->mxk)
                 hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(mgrid), dim3(256),
                                    mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a);
-            else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2 && fwd_grouped()) {
+            else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2) {
                 // the 2^gil groups that share workspace lines in one workgroup
                 MxArgs ag = a;
                 ag.gpw = 1 << ca.gil;
@@ -883,7 +775,7 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
                 ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
                 const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
                 const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
-                if (ag.gpw == 2 && fwd_mx16()) {
+                if (ag.gpw == 2) {
                     // groups with at most 32 degrees (m0 >= B - 32; 3/4 of them at l_max = 63) keep
                     // two 16-row tiles of D in registers instead of four: fewer registers, two
                     // workgroups per CU
@@ -899,7 +791,6 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
                         hipLaunchKernelGGL((contract_fwd_mx16g<2>), dim3(deal_grid(a2.nitems * a2.nsplit, a2.nsplit)),
                                            dim3(256), glds, s, a2);
                 }
-                else if (ag.gpw == 2) hipLaunchKernelGGL((contract_fwd_mx32g<2>), dim3(ggrid), dim3(256), glds, s, ag);
                 else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
             } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
             else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
@@ -935,18 +826,14 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             hipLaunchKernelGGL((contract_inv_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(2) + rcb, s, a);
         else if (c->mxk)
         {
-            if (mxk_inv8()) {
-                // 8 waves per workgroup over blocks of 256 columns: half as many units
-                MxArgs a8 = a;
-                const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
-                a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
-                hipLaunchKernelGGL((contract_inv_mxk<NP, 8>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
-                                   dim3(512), mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a8);
-            } else
-                hipLaunchKernelGGL((contract_inv_mxk<NP, 4>), dim3(mgrid), dim3(256),
-                                   mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a);
+            // 8 waves per workgroup over blocks of 256 columns: half as many units
+            MxArgs a8 = a;
+            const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+            a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
+            hipLaunchKernelGGL((contract_inv_mxk<NP, 8>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
+                               dim3(512), mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a8);
         }
-        else if (c->N == 128 && NP == 4 && ca.gil == 1 && inv_grouped()) {
+        else if (c->N == 128 && NP == 4 && ca.gil == 1) {
             // the two groups that share workspace lines in one workgroup
             MxArgs ag = a;
             ag.gpw = 2;
@@ -1052,7 +939,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             } else if (use_fft128<T>(c)) {
                 if constexpr (sizeof(T) == 4) {
                     size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
-                    if (ca.gil == 1 && fft128_pairs()) {
+                    if (ca.gil == 1) {
                         const int g2 = xcd_grid((nfc + 1) / 2 * N);
                         if (real) hipLaunchKernelGGL((so3_fft128_fwd2<true>), dim3(g2), dim3(1024), lds, s, a);
                         else hipLaunchKernelGGL((so3_fft128_fwd2<false>), dim3(g2), dim3(1024), lds, s, a);
@@ -1145,7 +1032,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             } else if (use_fft128<T>(c)) {
                 if constexpr (sizeof(T) == 4) {
                     size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
-                    if (ca.gil == 1 && fft128_pairs_inv()) {
+                    if (ca.gil == 1) {
                         const int g2 = xcd_grid((nfc + 1) / 2 * N);
                         if (real) hipLaunchKernelGGL((so3_fft128_inv2<true>), dim3(g2), dim3(1024), lds, s, a);
                         else hipLaunchKernelGGL((so3_fft128_inv2<false>), dim3(g2), dim3(1024), lds, s, a);
@@ -1189,9 +1076,7 @@ bool unified_eligible(const fftso3_plan_s *pl)
     for (const Class *c : pl->classes) {
         if (!c->mx || c->mxk) return false;
         if (c->N == 128) {
-            if (ws_gil(pl, c) != 1 || !fft128_pairs() || !fft128_pairs_inv() || !fwd_grouped() || !fwd_mx16() ||
-                !inv_grouped())
-                return false;
+            if (ws_gil(pl, c) != 1) return false;
         } else if (!c->r16) {
             return false;
         }
@@ -1521,7 +1406,6 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv<double, 4>));
     ALLOW_LDS((contract_inv<double, 8>));
     ALLOW_LDS((contract_fwd_mx32<4, true>));
-    ALLOW_LDS((contract_fwd_mx32g<2>));
     ALLOW_LDS((contract_fwd_mx32g<4>));
     ALLOW_LDS((contract_inv_mx32g<2>));
     ALLOW_LDS((contract_fwd_mx16g<4>));
@@ -1538,8 +1422,6 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv_tab<2>));
     ALLOW_LDS((contract_fwd_mxk<4>));
     ALLOW_LDS((contract_fwd_mxk<2>));
-    ALLOW_LDS((contract_inv_mxk<4, 4>));
-    ALLOW_LDS((contract_inv_mxk<2, 4>));
     ALLOW_LDS((contract_inv_mxk<4, 8>));
     ALLOW_LDS((contract_inv_mxk<2, 8>));
     ALLOW_LDS((contract_inv_mx32<2>));
@@ -1619,8 +1501,6 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
     pl->prec = precision;
     pl->n_fn = n_fn;
     pl->esz = precision == FFTSO3_FP64 ? 16 : 8;
-    pl->klo = ws_klo();
-    if (const char *e = getenv("FFTSO3_WS_GIL")) pl->gil = std::max(0, std::min(3, atoi(e)));
     pl->Bs.resize(n_fn);
     pl->grid_off.resize(n_fn);
     pl->coef_off.resize(n_fn);
@@ -1690,6 +1570,18 @@ int fftso3_plan_set_chunk(fftso3_plan_t plan, int chunk)
     if (chunk == plan->C && plan->force_chunked && plan->valid) return FFTSO3_OK;
     plan->C = chunk;
     plan->force_chunked = true;
+    return finalize_plan(plan);
+}
+
+int fftso3_plan_set_schedule(fftso3_plan_t plan, int klo, int gil, int split)
+{
+    if (!plan) return fail(FFTSO3_EINVAL, "plan is NULL");
+    if (klo < 0 || klo > 7) return fail(FFTSO3_EINVAL, "row blocking klo must be in [0, 7] (got %d)", klo);
+    if (gil < 0 || gil > 2) return fail(FFTSO3_EINVAL, "group interleave must be 0, 1 or 2 (got %d)", gil);
+    if (split != 1 && split != 2 && split != 4) return fail(FFTSO3_EINVAL, "split must be 1, 2 or 4 (got %d)", split);
+    plan->klo = klo;
+    plan->gil = gil;
+    plan->split_mul = split;
     return finalize_plan(plan);
 }
 

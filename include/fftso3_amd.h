@@ -104,6 +104,13 @@ int fftso3_plan_set_graph(fftso3_plan_t plan, int on);
  * are undefined afterwards.  No counterpart in the reference. */
 int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps);
 
+/* Schedule of the batch-wide kernels, the space autotune searches: workspace row blocking klo
+ * (0..7: rows [k >> klo][g][k & (2^klo - 1)]), group interleave gil (0..2: the records of 2^gil
+ * groups side by side; used at l_max = 63 with klo = 0) and group split (1, 2, 4: more
+ * workgroups per group when groups are few).  The defaults are 0, 1, 1.  No counterpart in the
+ * reference. */
+int fftso3_plan_set_schedule(fftso3_plan_t plan, int klo, int gil, int split);
+
 /* The plan's current schedule as text, one line per bandlimit class (l_max, functions, path,
  * functions per pass, workspace row blocking and group interleave, group split), written to buf
  * (at most len bytes, NUL-terminated).  No counterpart in the reference. */
