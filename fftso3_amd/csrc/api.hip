@@ -170,6 +170,8 @@ struct fftso3_plan_s {
     bool unified = false;              // in use for the current schedule
     DevBuf d_q_fft, d_q_cf, d_q_ci, d_q_tr;
     int nq_fft = 0, nq_cf = 0, nq_ci = 0, nq_tr = 0;
+    int ql_fft[XCD_COUNT] = {}, ql_cf[XCD_COUNT] = {}, ql_ci[XCD_COUNT] = {}, ql_tr[XCD_COUNT] = {};
+    DevBuf d_ctr;                      // queue counters of the four stage kernels (MULTI_CTRS each, 64 B apart)
     std::vector<Class *> uorder;                       // classes, largest bandlimit first
     std::vector<std::pair<Class *, int>> ent_cf, ent_ci;   // contraction entries (class, kind)
     // graph mode: one instantiated graph per (direction, input, output, stream)
@@ -1133,6 +1135,10 @@ struct XcdQueue {
             if (u < n) x[xx].push_back(make_int2(entry, u));
         }
     }
+    void lengths(int *len) const
+    {
+        for (int i = 0; i < XCD_COUNT; i++) len[i] = (int)x[i].size();
+    }
     std::vector<int2> flat() const
     {
         size_t L = 0;
@@ -1203,6 +1209,13 @@ int build_unified(fftso3_plan_s *pl)
     pl->nq_cf = (int)cf.size();
     pl->nq_ci = (int)ci.size();
     pl->nq_tr = (int)tr.size();
+    qf.lengths(pl->ql_fft);
+    qcf.lengths(pl->ql_cf);
+    qci.lengths(pl->ql_ci);
+    qtr.lengths(pl->ql_tr);
+    // the counters start at zero; each stage kernel's last workgroup resets its own
+    if ((st = pl->d_ctr.resize(4 * 16 * sizeof(unsigned)))) return st;
+    HIP_TRY(hipMemset(pl->d_ctr.p, 0, 4 * 16 * sizeof(unsigned)));
     return FFTSO3_OK;
 }
 
@@ -1234,8 +1247,9 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
             m.kind[i] = c->N == 128 ? 8 : c->r16;
             lds = std::max(lds, fft_lds(c));
         }
-        m.items = reinterpret_cast<const int2 *>(pl->d_q_fft.p);
-        m.nitems = pl->nq_fft;
+        m.q.items = reinterpret_cast<const int2 *>(pl->d_q_fft.p);
+        m.q.ctr = reinterpret_cast<unsigned *>(pl->d_ctr.p);
+        std::memcpy(m.q.len, pl->ql_fft, sizeof(m.q.len));
         if (f && real) hipLaunchKernelGGL((so3_fft_multi<true, true>), dim3(persistent_grid(so3_fft_multi<true, true>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
         else if (f) hipLaunchKernelGGL((so3_fft_multi<true, false>), dim3(persistent_grid(so3_fft_multi<true, false>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
         else if (real) hipLaunchKernelGGL((so3_fft_multi<false, true>), dim3(persistent_grid(so3_fft_multi<false, true>, lds, pl->nq_fft)), dim3(1024), lds, s, m);
@@ -1262,10 +1276,12 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
                                : (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
             lds = std::max(lds, l);
         }
-        m.items = reinterpret_cast<const int2 *>((f ? pl->d_q_cf : pl->d_q_ci).p);
-        m.nitems = f ? pl->nq_cf : pl->nq_ci;
-        if (f) hipLaunchKernelGGL(contract_fwd_multi, dim3(persistent_grid(contract_fwd_multi, lds, m.nitems, 256)), dim3(256), lds, s, m);
-        else hipLaunchKernelGGL(contract_inv_multi, dim3(persistent_grid(contract_inv_multi, lds, m.nitems, 256)), dim3(256), lds, s, m);
+        m.q.items = reinterpret_cast<const int2 *>((f ? pl->d_q_cf : pl->d_q_ci).p);
+        m.q.ctr = reinterpret_cast<unsigned *>(pl->d_ctr.p) + (f ? 16 : 32);
+        std::memcpy(m.q.len, f ? pl->ql_cf : pl->ql_ci, sizeof(m.q.len));
+        const int nitems = f ? pl->nq_cf : pl->nq_ci;
+        if (f) hipLaunchKernelGGL(contract_fwd_multi, dim3(persistent_grid(contract_fwd_multi, lds, nitems, 256)), dim3(256), lds, s, m);
+        else hipLaunchKernelGGL(contract_inv_multi, dim3(persistent_grid(contract_inv_multi, lds, nitems, 256)), dim3(256), lds, s, m);
         HIP_TRY(hipGetLastError());
         return FFTSO3_OK;
     };
@@ -1284,10 +1300,11 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
                 m.dst[i] = fc + c->fc_off;
             }
         }
-        m.items = reinterpret_cast<const int2 *>(pl->d_q_tr.p);
-        m.nitems = pl->nq_tr;
-        if (dir == 0) hipLaunchKernelGGL((coef_transpose_multi<0>), dim3(persistent_grid(coef_transpose_multi<0>, 0, m.nitems, 256)), dim3(256), 0, s, m);
-        else hipLaunchKernelGGL((coef_transpose_multi<1>), dim3(persistent_grid(coef_transpose_multi<1>, 0, m.nitems, 256)), dim3(256), 0, s, m);
+        m.q.items = reinterpret_cast<const int2 *>(pl->d_q_tr.p);
+        m.q.ctr = reinterpret_cast<unsigned *>(pl->d_ctr.p) + 48;
+        std::memcpy(m.q.len, pl->ql_tr, sizeof(m.q.len));
+        if (dir == 0) hipLaunchKernelGGL((coef_transpose_multi<0>), dim3(persistent_grid(coef_transpose_multi<0>, 0, pl->nq_tr, 256)), dim3(256), 0, s, m);
+        else hipLaunchKernelGGL((coef_transpose_multi<1>), dim3(persistent_grid(coef_transpose_multi<1>, 0, pl->nq_tr, 256)), dim3(256), 0, s, m);
         HIP_TRY(hipGetLastError());
         return FFTSO3_OK;
     };
@@ -1455,12 +1472,15 @@ int set_kernel_attrs()
     ALLOW_R16(4);
     ALLOW_R16(6);
 #undef ALLOW_R16
-    ALLOW_LDS((so3_fft_multi<true, true>));
-    ALLOW_LDS((so3_fft_multi<true, false>));
-    ALLOW_LDS((so3_fft_multi<false, true>));
-    ALLOW_LDS((so3_fft_multi<false, false>));
-    ALLOW_LDS(contract_fwd_multi);
-    ALLOW_LDS(contract_inv_multi);
+    // the queue kernels keep a few bytes of static LDS for their claims
+#define ALLOW_LDS_Q(k) HIP_TRY(hipFuncSetAttribute((const void *)(k), hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds - 1024))
+    ALLOW_LDS_Q((so3_fft_multi<true, true>));
+    ALLOW_LDS_Q((so3_fft_multi<true, false>));
+    ALLOW_LDS_Q((so3_fft_multi<false, true>));
+    ALLOW_LDS_Q((so3_fft_multi<false, false>));
+    ALLOW_LDS_Q(contract_fwd_multi);
+    ALLOW_LDS_Q(contract_inv_multi);
+#undef ALLOW_LDS_Q
     ALLOW_LDS((contract_fwd_mxs<4, 2>));
     ALLOW_LDS((contract_fwd_mxs<4, 3>));
     ALLOW_LDS((contract_fwd_mxs<2, 2>));

@@ -1,15 +1,16 @@
 // multi.hpp -- one launch per pipeline stage for every bandlimit class of an SO(3) batch (FP32,
 // batch-wide kernels, gfx950).
 //
-// Each stage is one persistent kernel over a static work queue of (class, unit) items that the
-// plan builds once: for every class, largest bandlimit first, the units its own launch would
-// have given each XCD (xcd_unit / xcd_deal), appended to that XCD's queue, and the 8 queues
-// interleaved so that queue position p belongs to XCD p % 8.  The grid is a multiple of 8 and
-// workgroup b takes positions b, b + grid, b + 2 grid, ...: it stays on XCD b % 8 and drains that
-// XCD's queue in order, so every XCD works through its even share of every class, long units
-// first, and the units that share workspace lines still meet in one L2.  A unit runs the same
-// device code as the per-class kernels (the *_body functions); a barrier separates consecutive
-// units of a workgroup, whose LDS layouts differ between classes.
+// Each stage is one persistent kernel over a work queue of (class, unit) items that the plan
+// builds once: for every class, largest bandlimit first, the units its own launch would have given
+// each XCD (xcd_unit / xcd_deal) are appended to that XCD's queue; queue x holds its q-th item at
+// position q * 8 + x.  Workgroup b belongs to XCD b % 8 (the hardware deals workgroups round-robin
+// over the XCDs) and claims the next item of that XCD's queue with an atomic counter until the
+// queue is empty, so every XCD works through its share of every class, long units first, the
+// units that share workspace lines still meet in one L2, and a slow workgroup takes fewer items.
+// The last workgroup to finish resets the counters for the next launch (stream order makes the
+// reset visible to it).  A unit runs the same device code as the per-class kernels (the *_body
+// functions); the claim's barrier also separates consecutive units, whose LDS layouts differ.
 #pragma once
 
 #include "fft_r16.hpp"
@@ -23,6 +24,50 @@ __device__ __forceinline__ int2 uniform_item(const int2 *items, int p)
     return make_int2(__builtin_amdgcn_readfirstlane(it.x), __builtin_amdgcn_readfirstlane(it.y));
 }
 
+// counters of one stage's queues: [XCD_COUNT] next position per XCD, then the finished workgroups
+#define MULTI_CTRS (XCD_COUNT + 1)
+
+// the next position of XCD x's queue, the same for the whole workgroup
+__device__ __forceinline__ int multi_claim(unsigned *ctr, int x)
+{
+    __shared__ int s_q;
+    __syncthreads();                                  // the previous claim read, the previous unit done
+    if (threadIdx.x == 0) s_q = (int)atomicAdd(ctr + x, 1u);
+    __syncthreads();
+    return s_q;
+}
+
+// every workgroup calls this once, after its last claim; the last one resets the counters
+__device__ __forceinline__ void multi_finish(unsigned *ctr)
+{
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(ctr + XCD_COUNT, 1u) == gridDim.x - 1) {
+            for (int x = 0; x <= XCD_COUNT; x++) atomicExch(ctr + x, 0u);
+        }
+    }
+}
+
+struct MultiQueue {
+    const int2 *items;     // (entry, unit) at position q * XCD_COUNT + x
+    unsigned *ctr;         // MULTI_CTRS counters, zero before the launch
+    int len[XCD_COUNT];    // items of each XCD's queue
+};
+
+// drain XCD (blockIdx.x % 8)'s queue, calling f(entry, unit) for every item
+template <typename F>
+__device__ __forceinline__ void multi_drain(const MultiQueue &q, F f)
+{
+    const int x = blockIdx.x % XCD_COUNT;
+    for (;;) {
+        const int p = multi_claim(q.ctr, x);
+        if (p >= q.len[x]) break;
+        const int2 it = uniform_item(q.items, p * XCD_COUNT + x);
+        if (it.x >= 0) f(it.x, it.y);
+    }
+    multi_finish(q.ctr);
+}
+
 #define MULTI_MAX_FFT 8     // classes of one FFT stage launch
 #define MULTI_MAX_MX 12     // contraction entries (a class may take two: grouped forward at N = 128)
 
@@ -30,30 +75,26 @@ __device__ __forceinline__ int2 uniform_item(const int2 *items, int p)
 struct MultiFft {
     FftArgs<float> c[MULTI_MAX_FFT];
     int kind[MULTI_MAX_FFT];
-    const int2 *items;     // (entry, unit); entry < 0: an empty slot of the queue
-    int nitems;
+    MultiQueue q;
 };
 
 template <bool FWD, bool REAL>
 __global__ void __launch_bounds__(1024) so3_fft_multi(MultiFft m)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    for (int p = blockIdx.x; p < m.nitems; p += gridDim.x) {
-        const int2 it = uniform_item(m.items, p);
-        __syncthreads();                      // the previous unit's LDS reads
-        if (it.x < 0) continue;
-        const FftArgs<float> &a = m.c[it.x];
-        switch (m.kind[it.x]) {
-        case 1: r16_item<1, FWD, REAL>(a, it.y, smem_raw); break;
-        case 2: r16_item<2, FWD, REAL>(a, it.y, smem_raw); break;
-        case 3: r16_item<3, FWD, REAL>(a, it.y, smem_raw); break;
-        case 4: r16_item<4, FWD, REAL>(a, it.y, smem_raw); break;
-        case 6: r16_item<6, FWD, REAL>(a, it.y, smem_raw); break;
+    multi_drain(m.q, [&](int e, int unit) {
+        const FftArgs<float> &a = m.c[e];
+        switch (m.kind[e]) {
+        case 1: r16_item<1, FWD, REAL>(a, unit, smem_raw); break;
+        case 2: r16_item<2, FWD, REAL>(a, unit, smem_raw); break;
+        case 3: r16_item<3, FWD, REAL>(a, unit, smem_raw); break;
+        case 4: r16_item<4, FWD, REAL>(a, unit, smem_raw); break;
+        case 6: r16_item<6, FWD, REAL>(a, unit, smem_raw); break;
         default:
-            if (FWD) so3_fft128_fwd2_body<REAL, true>(a, it.y);
-            else so3_fft128_inv2_body<REAL, true>(a, it.y);
+            if (FWD) so3_fft128_fwd2_body<REAL, true>(a, unit);
+            else so3_fft128_inv2_body<REAL, true>(a, unit);
         }
-    }
+    });
 }
 
 // contraction kinds: forward 0 = contract_fwd_mx32<4, false> (N < 128), 2 = contract_fwd_mx16g<4>,
@@ -62,35 +103,28 @@ __global__ void __launch_bounds__(1024) so3_fft_multi(MultiFft m)
 struct MultiMx {
     MxArgs c[MULTI_MAX_MX];
     int kind[MULTI_MAX_MX];
-    const int2 *items;
-    int nitems;
+    MultiQueue q;
 };
 
 __global__ void __launch_bounds__(256) contract_fwd_multi(MultiMx m)
 {
-    for (int p = blockIdx.x; p < m.nitems; p += gridDim.x) {
-        const int2 it = uniform_item(m.items, p);
-        __syncthreads();
-        if (it.x < 0) continue;
-        const MxArgs &a = m.c[it.x];
-        switch (m.kind[it.x]) {
-        case 2: contract_fwd_mx16g_body<4>(a, it.y); break;
-        case 3: contract_fwd_mx16g_body<2>(a, it.y); break;
-        default: contract_fwd_mx32_body<4, false>(a, it.y);
+    multi_drain(m.q, [&](int e, int unit) {
+        const MxArgs &a = m.c[e];
+        switch (m.kind[e]) {
+        case 2: contract_fwd_mx16g_body<4>(a, unit); break;
+        case 3: contract_fwd_mx16g_body<2>(a, unit); break;
+        default: contract_fwd_mx32_body<4, false>(a, unit);
         }
-    }
+    });
 }
 
 __global__ void __launch_bounds__(256) contract_inv_multi(MultiMx m)
 {
-    for (int p = blockIdx.x; p < m.nitems; p += gridDim.x) {
-        const int2 it = uniform_item(m.items, p);
-        __syncthreads();
-        if (it.x < 0) continue;
-        const MxArgs &a = m.c[it.x];
-        if (m.kind[it.x] == 1) contract_inv_mx32g_body<2>(a, it.y);
-        else contract_inv_mx32_body<4>(a, it.y);
-    }
+    multi_drain(m.q, [&](int e, int unit) {
+        const MxArgs &a = m.c[e];
+        if (m.kind[e] == 1) contract_inv_mx32g_body<2>(a, unit);
+        else contract_inv_mx32_body<4>(a, unit);
+    });
 }
 
 // coefficient layout passes (PT <-> packed complex) of every class
@@ -99,19 +133,19 @@ struct MultiTr {
     int ncoef[MULTI_MAX_FFT];
     const void *src[MULTI_MAX_FFT];
     void *dst[MULTI_MAX_FFT];
-    const int2 *items;
-    int nitems;
+    MultiQueue q;
 };
 
+// The layout tiles are small and of equal cost: workgroups stride through their XCD's queue
+// statically instead of claiming tiles one at a time (the counters are not used).
 template <int DIR>
 __global__ void __launch_bounds__(256) coef_transpose_multi(MultiTr m)
 {
-    for (int p = blockIdx.x; p < m.nitems; p += gridDim.x) {
-        const int2 it = uniform_item(m.items, p);
-        __syncthreads();
-        if (it.x < 0) continue;
-        const int c = it.x;
-        coef_transpose_body<DIR, float2>(m.ca[c], m.ncoef[c], m.src[c], m.dst[c], it.y);
+    const int x = blockIdx.x % XCD_COUNT, per = gridDim.x / XCD_COUNT;
+    for (int q = blockIdx.x / XCD_COUNT; q < m.q.len[x]; q += per) {
+        const int2 it = uniform_item(m.q.items, q * XCD_COUNT + x);
+        __syncthreads();                      // the previous tile's LDS reads
+        if (it.x >= 0) coef_transpose_body<DIR, float2>(m.ca[it.x], m.ncoef[it.x], m.src[it.x], m.dst[it.x], it.y);
     }
 }
 
