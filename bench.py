@@ -234,6 +234,12 @@ def bench_other(args, name):
     plan.timing(False)
     value = n_fn * args.steps / dt
     log("%s: %.1f transforms/s (%.3f ms/step)" % (name, value, 1e3 * dt / args.steps))
+    if args.calibrate_copy:
+        ca = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        cb = torch.empty_like(ca)
+        cb.copy_(ca)
+        torch.cuda.synchronize()
+        del ca, cb
     a = Fout.to(torch.complex128 if not real else torch.float64)
     b = F.to(a.dtype)
     rt = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))

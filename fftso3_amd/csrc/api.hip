@@ -584,7 +584,7 @@ template <typename T> bool use_s2fft256(const Class *c) { return sizeof(T) == 4 
 // quarter the contraction's.
 static int ws_gil(const fftso3_plan_s *pl, const Class *c)
 {
-    const bool ok = pl->prec == FFTSO3_FP32 && c->mx && !c->mxk && c->N == 128 && pl->klo == 0 &&
+    const bool ok = pl->prec != FFTSO3_FP64 && c->mx && !c->mxk && c->N == 128 && pl->klo == 0 &&
                     pl->domain == FFTSO3_DOMAIN_SO3;
     return ok ? pl->gil : 0;
 }
@@ -1817,7 +1817,8 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     } else {
         // split-BF16: the batch-wide kernels with each group split over 1, 2 or 4 workgroups, or
         // the chunked kernels (8 functions per pass)
-        for (int sp : {1, 2, 4}) cands.push_back({false, 8, 0, sp, 0});
+        for (int sp : {1, 2, 4})
+            for (int gil : {0, 1}) cands.push_back({false, 8, 0, sp, gil});
         cands.push_back({true, 8, 0, 1, 0});
     }
     if (cands.empty()) return FFTSO3_OK;

@@ -92,9 +92,9 @@ def summarize(stats_dir, fetch_dir, write_dir, copy_bytes):
     return {"fetch_scale": f_fetch, "write_scale": f_write, "copy_bytes": copy_bytes, "kernels": table}
 
 
-def add_mfma(s, mfma_dir):
-    """FP32 MFMA flops per launch (SQ_INSTS_VALU_MFMA_MOPS_F32 x 512) and their rate."""
-    mops = counter_per_dispatch(mfma_dir, "SQ_INSTS_VALU_MFMA_MOPS_F32")
+def add_mfma(s, mfma_dir, counter="SQ_INSTS_VALU_MFMA_MOPS_F32"):
+    """MFMA flops per launch (the MOPS counter counts units of 512 flops) and their rate."""
+    mops = counter_per_dispatch(mfma_dir, counter)
     for k, vs in mops.items():
         v = s["kernels"].get(short(k))
         if v is None:
@@ -103,10 +103,10 @@ def add_mfma(s, mfma_dir):
         v["mfma_f32_tflops"] = v["mfma_f32_flops"] / (v["avg_ms"] * 1e-3) / 1e12
 
 
-def to_markdown(s, command):
+def to_markdown(s, command, mfma_label="FP32"):
     mf = any("mfma_f32_flops" in v for v in s["kernels"].values())
     lines = ["| kernel | calls | avg ms | HBM read (GB) | HBM write (GB) | traffic (GB) |" +
-             (" FP32 MFMA GFLOP | MFMA TFLOP/s |" if mf else ""),
+             (" %s MFMA GFLOP | MFMA TFLOP/s |" % mfma_label if mf else ""),
              "|---|---|---|---|---|---|" + ("---|---|" if mf else "")]
     for k, v in sorted(s["kernels"].items(), key=lambda kv: -(kv[1]["avg_ms"] * max(kv[1]["calls"], 1))):
         row = "| %s | %d | %.3f | %.3f | %.3f | %.3f |" % (
@@ -120,6 +120,14 @@ def to_markdown(s, command):
     return "\n".join(head + lines) + "\n"
 
 
+def with_text(body, title, notes):
+    """The summary with a title line above and explanatory notes below."""
+    out = ("# " + title + "\n\n" if title else "") + body
+    if notes:
+        out += "\n" + notes.strip() + "\n"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("stats_dir")
@@ -130,18 +138,26 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--mfma-dir", default=None)
     ap.add_argument("--stats-md", default=None)
+    ap.add_argument("--mfma-counter", default="SQ_INSTS_VALU_MFMA_MOPS_F32")
+    ap.add_argument("--mfma-label", default="FP32")
+    ap.add_argument("--title", default="", help="title line of the traffic summary")
+    ap.add_argument("--stats-title", default="", help="title line of the --stats summary")
+    ap.add_argument("--notes", default="", help="file whose text goes below the traffic table")
+    ap.add_argument("--stats-notes", default="", help="file whose text goes below the --stats table")
     a = ap.parse_args()
+    read = lambda path: open(path).read() if path else ""
     s = summarize(a.stats_dir, a.fetch_dir, a.write_dir, a.copy_bytes)
     if a.mfma_dir:
-        add_mfma(s, a.mfma_dir)
+        add_mfma(s, a.mfma_dir, a.mfma_counter)
     if a.stats_md:
         with open(a.stats_md, "w") as f:
-            f.write(stats_markdown(a.stats_dir, a.command))
+            f.write(with_text(stats_markdown(a.stats_dir, a.command), a.stats_title, read(a.stats_notes)))
     with open(a.out_prefix + ".json", "w") as f:
         json.dump(s, f, indent=1)
+    md = with_text(to_markdown(s, a.command, a.mfma_label), a.title, read(a.notes))
     with open(a.out_prefix + ".md", "w") as f:
-        f.write(to_markdown(s, a.command))
-    print(to_markdown(s, a.command))
+        f.write(md)
+    print(md)
 
 
 if __name__ == "__main__":

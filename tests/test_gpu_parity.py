@@ -342,6 +342,8 @@ def test_split_bf16_batchwide(fftso3, torch_cuda, oracle, prec, domain, B, kind)
         F = inputs.s2_coefficients(B, n, 7000 + B, complex_=cplx, scale=lambda l: 1.0 / (1.0 + l))
     plan = fftso3.Plan(B - 1, n, domain, kind, prec)
     assert "batch-wide" in plan.describe()
+    if domain == "so3" and B == 64 and kind == "real":
+        plan.set_schedule(0, 0, 1)   # one group per workgroup and one-slice FFTs (the default interleaves 2)
     gdt, cdt = plan.dtypes()
     g = plan.inverse(_to_dev(torch, F, cdt))
     c = plan.forward(g)
