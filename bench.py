@@ -245,7 +245,8 @@ def bench_other(args, name):
     rt = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
     # FP32 round-trip bars of these workloads (the 512-point beta sums and DFTs at l_max = 255 in
     # FP32 lose about one more digit)
-    rt_bar = 1e-5 if name == "so3_b256x8" else 2e-6
+    # FP32-grade round trip (looser at l_max = 255); bf16x3 keeps its own 5e-5 bar
+    rt_bar = max(1e-5 if name == "so3_b256x8" else 2e-6, PARITY_BAR[args.precision])
     per_stage = workload_bytes(domain, Bs, real)
     dom = max(per_stage, key=lambda st: stages[st][0])
     avg_ms = stages[dom][0] / stages[dom][1]

@@ -487,9 +487,9 @@ int finalize_plan(fftso3_plan_s *pl)
     for (auto *c : pl->classes) {
         if ((st = build_class_tables(pl, c)) || (st = build_class_work(pl, c))) return st;
         if ((st = c->d_fn.upload(c->fns))) return st;
-        // N = 16R slice FFTs (fft_r16.hpp) for the FP32 batch-wide SO(3) classes of B = 8R, R <= 6
+        // N = 16R slice FFTs (fft_r16.hpp) for the float batch-wide SO(3) classes of B = 8R, R <= 6
         const int R = c->B / 8;
-        c->r16 = (pl->prec == FFTSO3_FP32 && !s2 && c->mx && !c->mxk && c->B % 8 == 0 &&
+        c->r16 = (pl->prec != FFTSO3_FP64 && !s2 && c->mx && !c->mxk && c->B % 8 == 0 &&
                   (R == 1 || R == 2 || R == 3 || R == 4 || R == 6)) ? R : 0;
     }
     pl->unified = unified_eligible(pl);
@@ -1072,7 +1072,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
 // ---------------------------------------------------------------------------------------------
 bool unified_eligible(const fftso3_plan_s *pl)
 {
-    if (!pl->multi_on || pl->prec != FFTSO3_FP32 || pl->domain != FFTSO3_DOMAIN_SO3 || pl->force_chunked ||
+    if (!pl->multi_on || pl->prec == FFTSO3_FP64 || pl->domain != FFTSO3_DOMAIN_SO3 || pl->force_chunked ||
         pl->klo != 0 || pl->classes.size() < 2 || pl->classes.size() > MULTI_MAX_FFT)
         return false;
     for (const Class *c : pl->classes) {
@@ -1187,16 +1187,18 @@ int build_unified(fftso3_plan_s *pl)
         const ChunkArgs ca = chunk_args<float>(pl, c, 0, nfc);
         qf.add_units(i, fft_units(c));
         qtr.add_units(i, coef_transpose_units<float2>(so3_ncoef(c->B), nfc));
-        const std::vector<int> kf = c->N == 128 ? std::vector<int>{2, 3} : std::vector<int>{0};
+        // split-BF16 plans run the one-group kernels (kind 0) at every bandlimit
+        const bool grouped = c->N == 128 && pl->prec == FFTSO3_FP32;
+        const std::vector<int> kf = grouped ? std::vector<int>{2, 3} : std::vector<int>{0};
         for (int kind : kf) {
             const MxArgs a = make_mx(pl, c, ca, true, kind);
             if (a.nitems <= 0) continue;
             qcf.add_dealt((int)pl->ent_cf.size(), a.nitems * a.nsplit, a.gpw > 1 ? a.nsplit : a.nsplit << ca.gil);
             pl->ent_cf.push_back({c, kind});
         }
-        const MxArgs ai = make_mx(pl, c, ca, false, c->N == 128 ? 1 : 0);
+        const MxArgs ai = make_mx(pl, c, ca, false, grouped ? 1 : 0);
         qci.add_dealt((int)pl->ent_ci.size(), ai.nitems * ai.nsplit, ai.gpw > 1 ? ai.nsplit : ai.nsplit << ca.gil);
-        pl->ent_ci.push_back({c, c->N == 128 ? 1 : 0});
+        pl->ent_ci.push_back({c, grouped ? 1 : 0});
     }
     if (pl->ent_cf.size() > MULTI_MAX_MX || pl->ent_ci.size() > MULTI_MAX_MX)
         return fail(FFTSO3_EUNSUPPORTED, "too many contraction entries for one launch");
@@ -1257,6 +1259,8 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
         HIP_TRY(hipGetLastError());
         return FFTSO3_OK;
     };
+    // split-BF16 pieces per operand (0: FP32 MFMA kernels)
+    const int npc = pl->prec == FFTSO3_BF16X6 ? 3 : pl->prec == FFTSO3_BF16X3 ? 2 : 0;
     auto contract_stage = [&](bool f) -> int {
         MultiMx m;
         std::memset(&m, 0, sizeof(m));
@@ -1271,7 +1275,8 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
             m.c[e] = a;
             m.kind[e] = ent[e].second;
             const size_t rcb = (size_t)c->B * sizeof(RecCoef);
-            const size_t l = ent[e].second == 0 ? MX_LDS_BYTES + rcb
+            const size_t l = npc ? (f ? mxs_fwd_lds_bytes(npc) : mxs_inv_lds_bytes(npc)) + rcb
+                           : ent[e].second == 0 ? MX_LDS_BYTES + rcb
                            : f ? (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb
                                : (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
             lds = std::max(lds, l);
@@ -1280,8 +1285,16 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
         m.q.ctr = reinterpret_cast<unsigned *>(pl->d_ctr.p) + (f ? 16 : 32);
         std::memcpy(m.q.len, f ? pl->ql_cf : pl->ql_ci, sizeof(m.q.len));
         const int nitems = f ? pl->nq_cf : pl->nq_ci;
-        if (f) hipLaunchKernelGGL(contract_fwd_multi, dim3(persistent_grid(contract_fwd_multi, lds, nitems, 256)), dim3(256), lds, s, m);
-        else hipLaunchKernelGGL(contract_inv_multi, dim3(persistent_grid(contract_inv_multi, lds, nitems, 256)), dim3(256), lds, s, m);
+#define MULTI_LAUNCH(K) hipLaunchKernelGGL(K, dim3(persistent_grid(K, lds, nitems, 256)), dim3(256), lds, s, m)
+        if (npc == 3) {
+            if (f) MULTI_LAUNCH(contract_fwd_multi_s<3>);
+            else MULTI_LAUNCH(contract_inv_multi_s<3>);
+        } else if (npc == 2) {
+            if (f) MULTI_LAUNCH(contract_fwd_multi_s<2>);
+            else MULTI_LAUNCH(contract_inv_multi_s<2>);
+        } else if (f) MULTI_LAUNCH(contract_fwd_multi);
+        else MULTI_LAUNCH(contract_inv_multi);
+#undef MULTI_LAUNCH
         HIP_TRY(hipGetLastError());
         return FFTSO3_OK;
     };
@@ -1480,6 +1493,10 @@ int set_kernel_attrs()
     ALLOW_LDS_Q((so3_fft_multi<false, false>));
     ALLOW_LDS_Q(contract_fwd_multi);
     ALLOW_LDS_Q(contract_inv_multi);
+    ALLOW_LDS_Q(contract_fwd_multi_s<3>);
+    ALLOW_LDS_Q(contract_inv_multi_s<3>);
+    ALLOW_LDS_Q(contract_fwd_multi_s<2>);
+    ALLOW_LDS_Q(contract_inv_multi_s<2>);
 #undef ALLOW_LDS_Q
     ALLOW_LDS((contract_fwd_mxs<4, 2>));
     ALLOW_LDS((contract_fwd_mxs<4, 3>));

@@ -48,7 +48,7 @@ __device__ __forceinline__ void split8(const float *x, bf16x8_t *out)
 
 // Forward: PT(packed(l, pair); fn) = s_p sum_k w_k d^l_{m0 n0}(beta_k) S_k(fn, pair), split-BF16.
 template <int NP, int NPC>
-__global__ void __launch_bounds__(256) contract_fwd_mxs(MxArgs a)
+__device__ __forceinline__ void contract_fwd_mxs_body(const MxArgs &a, int unit)
 {
     typedef SplitProducts<NPC> SP;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(256) contract_fwd_mxs(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, mx_block_unit(a), g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nrt = (M + 31) / 32, nt = (N + 15) / 16;
@@ -161,10 +161,16 @@ __global__ void __launch_bounds__(256) contract_fwd_mxs(MxArgs a)
     }
 }
 
+template <int NP, int NPC>
+__global__ void __launch_bounds__(256) contract_fwd_mxs(MxArgs a)
+{
+    contract_fwd_mxs_body<NP, NPC>(a, mx_block_unit(a));
+}
+
 // Inverse: G_k(fn, pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) PT(packed(l, pair); fn), split-BF16
 // (c_l = 2l+1 for SO(3), 1 for S^2).
 template <int NP, int NPC>
-__global__ void __launch_bounds__(256) contract_inv_mxs(MxArgs a)
+__device__ __forceinline__ void contract_inv_mxs_body(const MxArgs &a, int unit)
 {
     typedef SplitProducts<NPC> SP;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -175,7 +181,7 @@ __global__ void __launch_bounds__(256) contract_inv_mxs(MxArgs a)
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc * NP * 2 + 127) / 128;
     int g, fb0, fb1;
-    if (!mx_unit(a, nfb, mx_block_unit(a), g, fb0, fb1)) return;
+    if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
     const int M = B - m0, nu = (M + 15) / 16, nkt = (N + 31) / 32;
@@ -284,4 +290,10 @@ __global__ void __launch_bounds__(256) contract_inv_mxs(MxArgs a)
         if (fb + 2 < fb1) load_block(bA, fb + 2);
         multiply_store(bB, fb + 1);
     }
+}
+
+template <int NP, int NPC>
+__global__ void __launch_bounds__(256) contract_inv_mxs(MxArgs a)
+{
+    contract_inv_mxs_body<NP, NPC>(a, mx_block_unit(a));
 }

@@ -15,6 +15,7 @@
 
 #include "fft_r16.hpp"
 #include "mfma32.hpp"
+#include "mfma_split.hpp"
 
 // the queue entry at position p, in scalar registers: every lane of the workgroup reads the same
 // item, and its class index then selects kernel arguments with scalar loads
@@ -125,6 +126,19 @@ __global__ void __launch_bounds__(256) contract_inv_multi(MultiMx m)
         if (m.kind[e] == 1) contract_inv_mx32g_body<2>(a, unit);
         else contract_inv_mx32_body<4>(a, unit);
     });
+}
+
+// split-BF16 plans (NPC pieces per operand): every class's contraction is the one-group kernel
+template <int NPC>
+__global__ void __launch_bounds__(256) contract_fwd_multi_s(MultiMx m)
+{
+    multi_drain(m.q, [&](int e, int unit) { contract_fwd_mxs_body<4, NPC>(m.c[e], unit); });
+}
+
+template <int NPC>
+__global__ void __launch_bounds__(256) contract_inv_multi_s(MultiMx m)
+{
+    multi_drain(m.q, [&](int e, int unit) { contract_inv_mxs_body<4, NPC>(m.c[e], unit); });
 }
 
 // coefficient layout passes (PT <-> packed complex) of every class
