@@ -219,10 +219,22 @@ def bench_other(args, name):
         plan.inverse(F, grid, stream)
         plan.forward(grid, Fout, stream)
 
+    tuned = use_graph = False
+    if name == "so3_uneven512_real":
+        # as the configuration states it: tuned schedule, captured fwd + inv pipeline
+        tuned, use_graph = not args.no_autotune, not args.eager
+    if tuned:
+        t0 = time.perf_counter()
+        plan.autotune(grid, Fout, stream)   # clobbers both; F stays the input
+        log("autotuned in %.1f s: %s" % (time.perf_counter() - t0, " | ".join(plan.describe().strip().splitlines())))
+    if use_graph:
+        plan.set_graph(True)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    plan.timing(True)
+    # stage timing runs eagerly; with a graph the timed steps replay it and the stage breakdown
+    # comes from separate eager steps
+    plan.timing(not use_graph)
     plan.timing_read()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -231,9 +243,17 @@ def bench_other(args, name):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     stages = plan.timing_read()
+    if use_graph:
+        plan.timing(True)
+        plan.timing_read()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        stages = plan.timing_read()
     plan.timing(False)
     value = n_fn * args.steps / dt
-    log("%s: %.1f transforms/s (%.3f ms/step)" % (name, value, 1e3 * dt / args.steps))
+    log("%s: %.1f transforms/s (%.3f ms/step)%s" % (name, value, 1e3 * dt / args.steps,
+                                                     ", graph replays" if use_graph else ""))
     if args.calibrate_copy:
         ca = torch.empty(1 << 28, dtype=torch.float32, device=dev)
         cb = torch.empty_like(ca)
@@ -243,9 +263,8 @@ def bench_other(args, name):
     a = Fout.to(torch.complex128 if not real else torch.float64)
     b = F.to(a.dtype)
     rt = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
-    # FP32 round-trip bars of these workloads (the 512-point beta sums and DFTs at l_max = 255 in
-    # FP32 lose about one more digit)
-    # FP32-grade round trip (looser at l_max = 255); bf16x3 keeps its own 5e-5 bar
+    # FP32-grade round-trip bars (the 512-point beta sums and DFTs at l_max = 255 in FP32 lose
+    # about one more digit); bf16x3 keeps its own 5e-5 bar
     rt_bar = max(1e-5 if name == "so3_b256x8" else 2e-6, PARITY_BAR[args.precision])
     per_stage = workload_bytes(domain, Bs, real)
     dom = max(per_stage, key=lambda st: stages[st][0])
@@ -268,7 +287,7 @@ def bench_other(args, name):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": OTHER_WORKLOADS[name], "functions": n_fn, "precision": args.precision,
-                      "schedule": plan.describe().strip().splitlines()[0]},
+                      "schedule": plan.describe().strip().splitlines()[0], "graph": use_graph, "autotuned": tuned},
            "roofline": {"bound": "hbm", "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": bl,
                         "avg_launch_ms": avg_ms},
@@ -288,7 +307,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--precision", default=None, choices=["fp32", "bf16x3", "bf16x6"],
+                    help="default fp32; bf16x6 for so3_uneven512_real, whose configuration names split-BF16")
+    ap.add_argument("--eager", action="store_true", help="so3_uneven512_real: no graph capture")
+    ap.add_argument("--no-autotune", action="store_true", help="so3_uneven512_real: default schedule")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--chunk", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -301,6 +323,8 @@ def main():
     ap.add_argument("--workload", default="so3_b64x256", choices=["so3_b64x256"] + sorted(OTHER_WORKLOADS),
                     help="so3_b64x256 is the headline; the others print their own line")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "bf16x6" if args.workload == "so3_uneven512_real" else "fp32"
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)"
               % args.gpus, file=sys.stderr)
