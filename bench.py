@@ -188,28 +188,40 @@ def bench_other(args, name):
     from fftso3_amd import inputs
     from oracle import oracle
     dev = torch.device("cuda:0")
+    plan_s = []
+
+    def timed_plan(*a, **k):
+        t0 = time.perf_counter()
+        p = fftso3_amd.Plan(*a, **k)
+        torch.cuda.synchronize()
+        plan_s.append(time.perf_counter() - t0)
+        return p
+
     if name == "so3_uneven512_real":
         Bs, coefs = inputs.workload_uneven(512)
-        plan = fftso3_amd.Plan([B - 1 for B in Bs], domain="so3", kind="real", precision=args.precision,
-                               flags=fftso3_amd.FLAG_PER_CLASS_LAUNCHES if args.per_class_launches else 0)
+        plan = timed_plan([B - 1 for B in Bs], domain="so3", kind="real", precision=args.precision,
+                          flags=fftso3_amd.FLAG_PER_CLASS_LAUNCHES if args.per_class_launches else 0)
         F = torch.from_numpy(np.concatenate(coefs)).to(torch.float32).to(dev)
         domain, real = "so3", True
         ref_fn = lambda i: (oracle.so3_real_forward(Bs[i], oracle.so3_real_inverse(Bs[i], coefs[i], 0), 0))
     elif name == "so3_b256x8":
         Bs = [256] * 8
         Fh = inputs.workload_b256(8)
-        plan = fftso3_amd.Plan(255, 8, "so3", "complex", args.precision)
+        plan = timed_plan(255, 8, "so3", "complex", args.precision)
         F = torch.from_numpy(Fh.reshape(-1)).to(torch.complex64).to(dev)
         domain, real = "so3", False
         ref_fn = lambda i: oracle.so3_forward(256, oracle.so3_inverse(256, Fh[i], 0), 0)
     else:
         Bs = [128] * 4096
         Fh = inputs.workload_s2(4096)
-        plan = fftso3_amd.Plan(127, 4096, "s2", "complex", args.precision)
+        plan = timed_plan(127, 4096, "s2", "complex", args.precision)
         F = torch.from_numpy(Fh.reshape(-1)).to(torch.complex64).to(dev)
         domain, real = "s2", False
         ref_fn = lambda i: oracle.s2_forward(128, oracle.s2_inverse(128, Fh[i]))
     n_fn = len(Bs)
+    # plan creation: tables, work lists and (at l_max = 255 in FP32) the Wigner-d table
+    t_plan = plan_s[0]
+    log("plan created in %.2f s: %s" % (t_plan, " | ".join(plan.describe().strip().splitlines())))
     gdt, cdt = plan.dtypes()
     grid = torch.empty(plan.grid_elems(), dtype=gdt, device=dev)
     Fout = torch.empty_like(F)
@@ -293,6 +305,7 @@ def bench_other(args, name):
                         "avg_launch_ms": avg_ms},
            "cpu_baseline": cpu,
            "extra": {"roundtrip_rel_l2": rt, "roundtrip_bar": rt_bar, "parity_ok": rt <= rt_bar,
+                     "plan_create_s": t_plan, "plan_workspace_bytes": plan.workspace_bytes(),
                      "stage_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
                      "stage_launches_per_step": {k: v[1] / args.steps for k, v in stages.items()},
                      "pipeline_alg_GBs": sum(per_stage.values()) * value / n_fn / 1e9}}

@@ -67,10 +67,11 @@ enum fftso3_precision {
 
 /* --- plans ----------------------------------------------------------------------------------- */
 
-/* Plan flags.  By default an FP32 SO(3) plan of several bandlimits (each a multiple of 8, up to
- * l_max = 63) runs each pipeline stage as one persistent launch over all its bandlimit classes,
- * draining a static per-XCD queue of (class, unit) items, largest bandlimit first;
- * FFTSO3_FLAG_PER_CLASS_LAUNCHES asks for one launch per class and stage instead (same results). */
+/* Plan flags.  By default a float-storage (FP32 or split-BF16) SO(3) plan of several bandlimits
+ * (each a multiple of 8, up to l_max = 63) runs each pipeline stage as one persistent launch over
+ * all its bandlimit classes: workgroups claim (class, unit) items, largest bandlimit first, from
+ * per-XCD queues with atomic counters; FFTSO3_FLAG_PER_CLASS_LAUNCHES asks for one launch per
+ * class and stage instead (the same results to 1e-6 relative). */
 #define FFTSO3_FLAG_PER_CLASS_LAUNCHES 1u
 
 /* Uniform batch: n_fn functions of bandlimit B = l_max + 1.  flags: 0 or the flags above. */
