@@ -5,6 +5,7 @@
 #include "mfma32.hpp"
 #include "mfma_split.hpp"
 #include "multi.hpp"
+#include "mfma64.hpp"
 
 #include "../../include/fftso3_amd.h"
 
@@ -123,6 +124,9 @@ struct Class {
     // recursion tables
     DevBuf d_cb, d_lnc, d_lns, d_w, d_lnbin, d_sq, d_isq, d_lcoef;
     DevBuf d_items_fwd, d_items_inv;
+    bool mx64 = false;                       // batch-wide FP64 MFMA contractions (mfma64.hpp)
+    DevBuf d_items_fwd64, d_items_inv64;
+    int nitems_fwd64 = 0, nitems_inv64 = 0, kt64 = 0;
     int nitems_fwd = 0, nitems_inv = 0, lt_fwd = 0, kt_inv = 0;
     DevBuf d_items_fwd_mx, d_items_inv_mx;   // MFMA kernels: 64-row and 128-k tiles
     int nitems_fwd_mx = 0, nitems_inv_mx = 0;
@@ -343,6 +347,16 @@ RecTables rec_tables(Class *c)
     return rt;
 }
 
+// FP64 SO(3) batches (a plan of 5 or more functions per class, so 8 per pass by default) at
+// B <= 64 run batch-wide: the whole class per pass, contractions on FP64 MFMA (mfma64.hpp; the D
+// and coefficient blocks of larger B exceed the LDS its kernels use).  set_chunk() keeps the
+// chunked vector kernels.
+static bool mx64_eligible(const fftso3_plan_s *pl, const Class *c)
+{
+    return pl->prec == FFTSO3_FP64 && pl->domain == FFTSO3_DOMAIN_SO3 && c->B <= 64 && pl->C >= MX64_FB &&
+           !pl->force_chunked;
+}
+
 int build_class_work(fftso3_plan_s *pl, Class *c)
 {
     const int B = c->B, N = c->N;
@@ -357,7 +371,8 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
         c->mxk = pl->prec == FFTSO3_FP32 && N > 128 && fits && !pl->force_chunked;
         c->mx = ((pl->prec == FFTSO3_FP32 || split) && N <= 128 && !pl->force_chunked) || c->mxk;
     }
-    c->chunk = c->mx ? (int)c->fns.size() : pl->C;
+    c->mx64 = mx64_eligible(pl, c);
+    c->chunk = c->mx || c->mx64 ? (int)c->fns.size() : pl->C;
     // large bandlimits on the chunked kernels: coefficients go through PT[e][fi] so the contraction
     // moves whole rows of the chunk's functions instead of scattered single values
     c->pt = !c->mx && pl->kind == FFTSO3_COMPLEX && B > 64 && pl->prec != FFTSO3_BF16X3 &&
@@ -420,6 +435,22 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     c->nitems_fwd = (int)itf.size();
     c->nitems_inv = (int)iti.size();
     if ((st = c->d_items_fwd.upload(itf)) || (st = c->d_items_inv.upload(iti))) return st;
+    if (c->mx64) {
+        // FP64 MFMA contractions: 64-degree forward items (a whole group at B <= 64), inverse k
+        // tiles of up to 64
+        std::vector<int2> f64, i64;
+        c->kt64 = std::min(64, N);
+        for (int g = 0; g < ngrp; g++) {
+            int m0 = (int)std::sqrt((double)g + 0.5);
+            while (m0 * m0 > g) m0--;
+            while ((m0 + 1) * (m0 + 1) <= g) m0++;
+            for (int l = m0; l < B; l += 16 * MX64_RT) f64.push_back(make_int2(g, l));
+            for (int k = 0; k < N; k += c->kt64) i64.push_back(make_int2(g, k));
+        }
+        c->nitems_fwd64 = (int)f64.size();
+        c->nitems_inv64 = (int)i64.size();
+        if ((st = c->d_items_fwd64.upload(f64)) || (st = c->d_items_inv64.upload(i64))) return st;
+    }
     std::vector<int2> mf, mi;
     for (int g = 0; g < ngrp; g++) {
         int m0 = s2 ? g : (int)std::sqrt((double)g + 0.5);
@@ -496,7 +527,9 @@ int finalize_plan(fftso3_plan_s *pl)
     for (auto *c : pl->classes) {
         const size_t B = c->B, N = c->N, C = c->chunk;
         const int klo = pl->klo;
-        const size_t w = (size_t)ws_rows(c->mx, klo, s2 ? (int)B : (int)(B * B), (int)N) * C * (s2 ? 2 : 4) * pl->esz;
+        // (batch-wide FP64: whole blocks of 8 functions)
+        const size_t Cw = c->mx64 ? (C + MX64_FB - 1) / MX64_FB * MX64_FB : C;
+        const size_t w = (size_t)ws_rows(c->mx, klo, s2 ? (int)B : (int)(B * B), (int)N) * Cw * (s2 ? 2 : 4) * pl->esz;
         const size_t cf = C * (size_t)(s2 ? B * B : so3_ncoef((int)B)) * pl->esz;
         if (pl->unified) {
             // every class in a region of its own, 256-byte aligned
@@ -510,7 +543,8 @@ int finalize_plan(fftso3_plan_s *pl)
         }
         c->ws_off = c->fc_off = c->tmp_off = 0;
         ws = std::max(ws, w);
-        if (!s2 && !c->fused) scr = std::max(scr, C * N * N * N * pl->esz);
+        // two-pass FFT scratch: a pass's functions (batch-wide FP64: 8 at a time, fft_pass)
+        if (!s2 && !c->fused) scr = std::max(scr, (c->mx64 ? std::min<size_t>(C, MX64_FB) : C) * N * N * N * pl->esz);
         if (pl->kind == FFTSO3_REAL) fc = std::max(fc, cf);
         if (c->mx || c->pt) {
             fc = std::max(fc, cf);  // PT
@@ -851,9 +885,42 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     return FFTSO3_OK;
 }
 
+// batch-wide FP64 MFMA contractions (mfma64.hpp)
+int launch_contract_mx64(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst,
+                         hipStream_t s)
+{
+    ConArgs<double> a;
+    std::memset(&a, 0, sizeof(a));
+    a.ca = ca;
+    a.rt = rec_tables(c);
+    a.s2 = 0;
+    a.pt = c->pt ? 1 : 0;
+    a.dtab = c->d_dtab.p;
+    a.dtab_off = (const long *)c->d_dtab_off.p;
+    a.src = src;
+    a.dst = dst;
+    a.items = (const int2 *)(fwd ? c->d_items_fwd64.p : c->d_items_inv64.p);
+    a.nitems = fwd ? c->nitems_fwd64 : c->nitems_inv64;
+    a.tile = fwd ? 16 * MX64_RT : c->kt64;
+    // several workgroups share an item's blocks of 8 functions when items are few
+    const int nfb = (ca.nfc + MX64_FB - 1) / MX64_FB;
+    a.nsplit = std::max(1, std::min(nfb, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
+    if (fwd)
+        hipLaunchKernelGGL(contract_fwd_mx64, dim3(xcd_grid(a.nitems * a.nsplit)), dim3(MX64_THREADS),
+                           mx64_fwd_lds_bytes(c->N), s, a);
+    else
+        hipLaunchKernelGGL(contract_inv_mx64, dim3(deal_grid(a.nitems * a.nsplit, (c->N + c->kt64 - 1) / c->kt64)),
+                           dim3(MX64_THREADS), mx64_inv_lds_bytes(c->B, c->kt64), s, a);
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
 template <typename T>
 int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 8) {
+        if (c->mx64) return launch_contract_mx64(pl, c, ca, fwd, src, dst, s);
+    }
     if constexpr (sizeof(T) == 4) {
         if (pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6) {
             ConArgs<float> a;
@@ -904,6 +971,134 @@ int launch_r16(int R, bool fwd, bool real, const FftArgs<float> &a, hipStream_t 
     }
 }
 
+// The alpha/gamma FFTs of one pass: grid -> workspace (forward) or workspace -> grid (inverse)
+template <typename T>
+int fft_stage(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
+{
+    typedef typename cplx_t<T>::type T2;
+    const bool real = pl->kind == FFTSO3_REAL;
+    const bool s2 = pl->domain == FFTSO3_DOMAIN_S2;
+    const int N = c->N, nfc = ca.nfc;
+    const size_t tables = (size_t)N * (sizeof(T2) + sizeof(int));
+    int st = FFTSO3_OK;
+    (void)st;
+    if (fwd) {
+        FftArgs<T> a = fft_args<T>(pl, c, ca, src, dst);
+        if (s2 && use_s2fft256<T>(c)) {
+            if constexpr (sizeof(T) == 4) {
+                const int g256 = xcd_grid(((nfc + S2F_FNS - 1) / S2F_FNS) * (256 / S2F_KS));
+                if (real) hipLaunchKernelGGL((s2_fft256_fwd<true>), dim3(g256), dim3(256), 0, s, a);
+                else hipLaunchKernelGGL((s2_fft256_fwd<false>), dim3(g256), dim3(256), 0, s, a);
+            }
+        } else if (s2) {
+            a.tile = c->row_tile;
+            int nunit = nfc * (N / a.tile);
+            size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
+            if (real) hipLaunchKernelGGL((s2_fft_fwd<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            else hipLaunchKernelGGL((s2_fft_fwd<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+        } else if (use_fft128<T>(c)) {
+            if constexpr (sizeof(T) == 4) {
+                size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
+                if (ca.gil == 1) {
+                    const int g2 = xcd_grid((nfc + 1) / 2 * N);
+                    if (real) hipLaunchKernelGGL((so3_fft128_fwd2<true>), dim3(g2), dim3(1024), lds, s, a);
+                    else hipLaunchKernelGGL((so3_fft128_fwd2<false>), dim3(g2), dim3(1024), lds, s, a);
+                } else if (real) hipLaunchKernelGGL((so3_fft128_fwd<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft128_fwd<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+            }
+        } else if (c->r16) {
+            if constexpr (sizeof(T) == 4) {
+                if ((st = launch_r16(c->r16, true, real, a, s))) return st;
+            }
+        } else if (c->fused) {
+            size_t lds = (size_t)N * N * sizeof(T2) + tables;
+            if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+            else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+        } else {
+            a.tile = c->so3_tile;
+            int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
+            size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
+            if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            else hipLaunchKernelGGL((so3_fft_rows<T, false, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+            HIP_TRY(hipGetLastError());
+            a.tile = c->so3_tile;
+            nunit = nfc * N * ((N + a.tile - 1) / a.tile);
+            lds = (size_t)a.tile * N * sizeof(T2) + tables;
+            hipLaunchKernelGGL((so3_fft_cols<T, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+        }
+        HIP_TRY(hipGetLastError());
+        return FFTSO3_OK;
+    }
+    FftArgs<T> a = fft_args<T>(pl, c, ca, src, dst);
+    if (s2 && use_s2fft256<T>(c)) {
+        if constexpr (sizeof(T) == 4) {
+            const int g256 = xcd_grid(((nfc + S2F_FNS - 1) / S2F_FNS) * (256 / S2F_KS));
+            if (real) hipLaunchKernelGGL((s2_fft256_inv<true>), dim3(g256), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((s2_fft256_inv<false>), dim3(g256), dim3(256), 0, s, a);
+        }
+    } else if (s2) {
+        a.tile = c->row_tile;
+        int nunit = nfc * (N / a.tile);
+        size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
+        if (real) hipLaunchKernelGGL((s2_fft_inv<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+        else hipLaunchKernelGGL((s2_fft_inv<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+    } else if (use_fft128<T>(c)) {
+        if constexpr (sizeof(T) == 4) {
+            size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
+            if (ca.gil == 1) {
+                const int g2 = xcd_grid((nfc + 1) / 2 * N);
+                if (real) hipLaunchKernelGGL((so3_fft128_inv2<true>), dim3(g2), dim3(1024), lds, s, a);
+                else hipLaunchKernelGGL((so3_fft128_inv2<false>), dim3(g2), dim3(1024), lds, s, a);
+            } else if (real) hipLaunchKernelGGL((so3_fft128_inv<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+            else hipLaunchKernelGGL((so3_fft128_inv<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
+        }
+    } else if (c->r16) {
+        if constexpr (sizeof(T) == 4) {
+            if ((st = launch_r16(c->r16, false, real, a, s))) return st;
+        }
+    } else if (c->fused) {
+        size_t lds = (size_t)N * N * sizeof(T2) + tables;
+        if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+        else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+    } else {
+        a.tile = c->so3_tile;
+        int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
+        size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
+        hipLaunchKernelGGL((so3_fft_cols<T, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+        HIP_TRY(hipGetLastError());
+        a.tile = c->so3_tile;
+        nunit = nfc * N * ((N + a.tile - 1) / a.tile);
+        lds = (size_t)a.tile * N * sizeof(T2) + tables;
+        if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+        else hipLaunchKernelGGL((so3_fft_rows<T, false, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
+// The FFTs of a pass.  A batch-wide FP64 class (mfma64.hpp) keeps its workspace block-major (rows
+// of 8 functions, one region per block of 8) and runs its FFTs one block at a time, each into its
+// own region exactly as a chunk of 8 would; the two-pass FFTs' scratch grid between the row and
+// column passes (8 N^3 complex doubles, 268 MB at l_max = 63) is then also reused in the Infinity
+// Cache instead of making a round trip to HBM.
+template <typename T>
+int fft_pass(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
+{
+    typedef typename cplx_t<T>::type T2;
+    if (!c->mx64) return fft_stage<T>(pl, c, ca, fwd, src, dst, s);
+    for (int f0 = 0; f0 < ca.nfc; f0 += MX64_FB) {
+        ChunkArgs cs = ca;
+        cs.C = MX64_FB;
+        cs.nfc = std::min(MX64_FB, ca.nfc - f0);
+        cs.fn = ca.fn + f0;
+        const size_t wo = (size_t)(f0 / MX64_FB) * c->B * c->B * c->N * MX64_FB * 4 * sizeof(T2);   // block's region
+        int st = fwd ? fft_stage<T>(pl, c, cs, true, src, static_cast<char *>(dst) + wo, s)
+                     : fft_stage<T>(pl, c, cs, false, static_cast<const char *>(src) + wo, dst, s);
+        if (st) return st;
+    }
+    return FFTSO3_OK;
+}
+
 template <typename T>
 int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void *in, void *out, hipStream_t s)
 {
@@ -925,49 +1120,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
     if (fwd) {
         {
             StageTimer t(pl, s, 0);
-            FftArgs<T> a = fft_args<T>(pl, c, ca, in, pl->d_ws.p);
-            if (s2 && use_s2fft256<T>(c)) {
-                if constexpr (sizeof(T) == 4) {
-                    const int g256 = xcd_grid(((nfc + S2F_FNS - 1) / S2F_FNS) * (256 / S2F_KS));
-                    if (real) hipLaunchKernelGGL((s2_fft256_fwd<true>), dim3(g256), dim3(256), 0, s, a);
-                    else hipLaunchKernelGGL((s2_fft256_fwd<false>), dim3(g256), dim3(256), 0, s, a);
-                }
-            } else if (s2) {
-                a.tile = c->row_tile;
-                int nunit = nfc * (N / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((s2_fft_fwd<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((s2_fft_fwd<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-            } else if (use_fft128<T>(c)) {
-                if constexpr (sizeof(T) == 4) {
-                    size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
-                    if (ca.gil == 1) {
-                        const int g2 = xcd_grid((nfc + 1) / 2 * N);
-                        if (real) hipLaunchKernelGGL((so3_fft128_fwd2<true>), dim3(g2), dim3(1024), lds, s, a);
-                        else hipLaunchKernelGGL((so3_fft128_fwd2<false>), dim3(g2), dim3(1024), lds, s, a);
-                    } else if (real) hipLaunchKernelGGL((so3_fft128_fwd<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
-                    else hipLaunchKernelGGL((so3_fft128_fwd<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
-                }
-            } else if (c->r16) {
-                if constexpr (sizeof(T) == 4) {
-                    if ((st = launch_r16(c->r16, true, real, a, s))) return st;
-                }
-            } else if (c->fused) {
-                size_t lds = (size_t)N * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
-            } else {
-                a.tile = c->so3_tile;
-                int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_rows<T, false, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-                HIP_TRY(hipGetLastError());
-                a.tile = c->so3_tile;
-                nunit = nfc * N * ((N + a.tile - 1) / a.tile);
-                lds = (size_t)a.tile * N * sizeof(T2) + tables;
-                hipLaunchKernelGGL((so3_fft_cols<T, 0>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-            }
+            if ((st = fft_pass<T>(pl, c, ca, true, in, pl->d_ws.p, s))) return st;
             HIP_TRY(hipGetLastError());
         }
         if constexpr (sizeof(T) == 4) {
@@ -1018,49 +1171,7 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
         }
         {
             StageTimer t(pl, s, 5);
-            FftArgs<T> a = fft_args<T>(pl, c, ca, pl->d_ws.p, out);
-            if (s2 && use_s2fft256<T>(c)) {
-                if constexpr (sizeof(T) == 4) {
-                    const int g256 = xcd_grid(((nfc + S2F_FNS - 1) / S2F_FNS) * (256 / S2F_KS));
-                    if (real) hipLaunchKernelGGL((s2_fft256_inv<true>), dim3(g256), dim3(256), 0, s, a);
-                    else hipLaunchKernelGGL((s2_fft256_inv<false>), dim3(g256), dim3(256), 0, s, a);
-                }
-            } else if (s2) {
-                a.tile = c->row_tile;
-                int nunit = nfc * (N / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((s2_fft_inv<T, true>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((s2_fft_inv<T, false>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-            } else if (use_fft128<T>(c)) {
-                if constexpr (sizeof(T) == 4) {
-                    size_t lds = (size_t)F128 * P128 * sizeof(float2) + F128 * sizeof(float2);
-                    if (ca.gil == 1) {
-                        const int g2 = xcd_grid((nfc + 1) / 2 * N);
-                        if (real) hipLaunchKernelGGL((so3_fft128_inv2<true>), dim3(g2), dim3(1024), lds, s, a);
-                        else hipLaunchKernelGGL((so3_fft128_inv2<false>), dim3(g2), dim3(1024), lds, s, a);
-                    } else if (real) hipLaunchKernelGGL((so3_fft128_inv<true>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
-                    else hipLaunchKernelGGL((so3_fft128_inv<false>), dim3(xcd_grid(nfc * N)), dim3(1024), lds, s, a);
-                }
-            } else if (c->r16) {
-                if constexpr (sizeof(T) == 4) {
-                    if ((st = launch_r16(c->r16, false, real, a, s))) return st;
-                }
-            } else if (c->fused) {
-                size_t lds = (size_t)N * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
-            } else {
-                a.tile = c->so3_tile;
-                int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
-                size_t lds = (size_t)a.tile * N * sizeof(T2) + tables;
-                hipLaunchKernelGGL((so3_fft_cols<T, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-                HIP_TRY(hipGetLastError());
-                a.tile = c->so3_tile;
-                nunit = nfc * N * ((N + a.tile - 1) / a.tile);
-                lds = (size_t)a.tile * N * sizeof(T2) + tables;
-                if (real) hipLaunchKernelGGL((so3_fft_rows<T, true, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-                else hipLaunchKernelGGL((so3_fft_rows<T, false, 1>), dim3(xcd_grid(nunit)), dim3(kFftThreads), lds, s, a);
-            }
+            if ((st = fft_pass<T>(pl, c, ca, false, pl->d_ws.p, out, s))) return st;
             HIP_TRY(hipGetLastError());
         }
     }
@@ -1459,6 +1570,8 @@ int set_kernel_attrs()
     ALLOW_LDS((so3_fft128_fwd<false>));
     ALLOW_LDS((so3_fft128_inv<true>));
     ALLOW_LDS((so3_fft128_inv<false>));
+    ALLOW_LDS(contract_fwd_mx64);
+    ALLOW_LDS(contract_inv_mx64);
     ALLOW_LDS((so3_fft128_fwd2<true>));
     ALLOW_LDS((so3_fft128_fwd2<false>));
     ALLOW_LDS((so3_fft128_inv2<true>));
@@ -1632,10 +1745,10 @@ int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len)
         const char *path = c->mxk ? "batch-wide mfma (k-chunked)" : c->mx ? "batch-wide mfma" : "chunked";
         char line[256];
         ChunkArgs ca = chunk_args<float>(plan, const_cast<Class *>(c), 0, (int)c->fns.size());
-        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s\n", c->B - 1,
+        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s%s\n", c->B - 1,
                       (int)c->fns.size(), prec, path, c->chunk, c->mx ? plan->klo : 0, ca.gil, plan->split_mul,
                       c->r16 ? " fft16R" : "", plan->unified ? " unified" : "", c->d_dtab.p ? " d-table" : "",
-                      plan->graph ? " graph" : "");
+                      c->mx64 ? " mfma64" : "", plan->graph ? " graph" : "");
         out += line;
     }
     std::snprintf(buf, (size_t)len, "%s", out.c_str());
@@ -1830,6 +1943,9 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         // several bandlimit classes: also one launch per class and stage instead of the unified launches
         if (plan->prec == FFTSO3_FP32 && plan->classes.size() > 1)
             for (int sp : {1, 2}) cands.push_back({false, plan->C, 0, sp, 1, false});
+        // FP64: the batch-wide MFMA contractions with groups split over 1, 2 or 4 workgroups
+        if (plan->prec == FFTSO3_FP64)
+            for (int sp : {1, 2, 4}) cands.push_back({false, plan->C, 0, sp, 0});
         for (int C : {1, 2, 4, 8}) cands.push_back({true, C, plan->klo, 1, plan->gil});
     } else {
         // split-BF16: the batch-wide kernels with each group split over 1, 2 or 4 workgroups, or

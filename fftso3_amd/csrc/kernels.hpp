@@ -572,6 +572,7 @@ template <typename T> struct ConArgs {
     int pt;               // coefficients in PT[e][fi] (rows of C complex values) instead of packed
     const void *dtab;     // precomputed d^l_{m0 n0}(beta_k) per group ([l - m0][k], storage type), or null
     const long *dtab_off; // first element of each group's block in dtab
+    int nsplit;           // FP64 MFMA kernels: workgroups sharing an item's blocks of 8 functions
 };
 
 // Table of d^l_{m0 n0}(beta_k) (S^2: nP^l_{m0}(cos theta_k)) for every group, k and l >= m0, laid out

@@ -136,6 +136,32 @@ def test_so3_fp32_b64_headline_batch(fftso3, torch_cuda, oracle):
         assert rel_l2(gv[i].cpu().numpy(), g_ref) < FP32_TOL_B64, i
 
 
+@pytest.mark.parametrize("B,kind", [(2, "complex"), (7, "complex"), (16, "real"), (33, "complex"), (64, "complex"),
+                                    (64, "real")])
+def test_so3_fp64_mfma_contractions(fftso3, torch_cuda, oracle, B, kind):
+    """FP64 batches of 8 functions per pass contract on FP64 MFMA (mfma64.hpp): 11 functions (a full
+    pass and a tail of 3), every function round trips at 1e-10 and the first and last match the
+    oracle's inverse and forward."""
+    n = 11
+    cplx = kind == "complex"
+    F = inputs.so3_coefficients(B, n, 700 + B, complex_=cplx)
+    torch = torch_cuda
+    plan = fftso3.Plan(B - 1, n, "so3", kind, "fp64")
+    assert "mfma64" in plan.describe(), plan.describe()
+    gdt, cdt = plan.dtypes()
+    g = plan.inverse(_to_dev(torch, F, cdt))
+    c = plan.forward(g)
+    g = g.cpu().numpy().reshape(n, -1)
+    c = c.cpu().numpy().reshape(n, -1)
+    for i in range(n):
+        assert rel_l2(c[i], F[i]) < FP64_TOL, i
+    for i in (0, n - 1):
+        g_ref = oracle.so3_inverse(B, F[i]) if cplx else oracle.so3_real_inverse(B, F[i])
+        assert rel_l2(g[i], g_ref.reshape(-1)) < FP64_TOL, i
+        c_ref = oracle.so3_forward(B, g_ref) if cplx else oracle.so3_real_forward(B, g_ref)
+        assert rel_l2(c[i], c_ref) < FP64_TOL, i
+
+
 def test_so3_chunk_tail(fftso3, torch_cuda, oracle):
     """5 functions through chunks of 4 (a full chunk and a tail of 1)."""
     B = 6
