@@ -6,6 +6,7 @@
 #include "mfma_split.hpp"
 #include "multi.hpp"
 #include "mfma64.hpp"
+#include "fft_lines.hpp"
 
 #include "../../include/fftso3_amd.h"
 
@@ -125,6 +126,7 @@ struct Class {
     DevBuf d_cb, d_lnc, d_lns, d_w, d_lnbin, d_sq, d_isq, d_lcoef;
     DevBuf d_items_fwd, d_items_inv;
     bool mx64 = false;                       // batch-wide FP64 MFMA contractions (mfma64.hpp)
+    bool prfft = false;                      // two-pass FFTs of fft_lines.hpp (workspace in half records)
     DevBuf d_items_fwd64, d_items_inv64;
     int nitems_fwd64 = 0, nitems_inv64 = 0, kt64 = 0;
     int nitems_fwd = 0, nitems_inv = 0, lt_fwd = 0, kt_inv = 0;
@@ -488,6 +490,10 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     int rt3 = 1;
     while (rt3 * 2 <= N && (size_t)(rt3 * 2) * N <= cap) rt3 *= 2;
     c->so3_tile = rt3;
+    // slices larger than LDS whose contraction reads half records: FP64 l_max = 63 (batch-wide MFMA)
+    // and FP32 l_max = 255 (d table)
+    c->prfft = !s2 && ((pl->prec == FFTSO3_FP64 && c->mx64 && N == 128) ||
+                       (pl->prec == FFTSO3_FP32 && !c->mx && c->d_dtab.p && N == 512));
     return FFTSO3_OK;
 }
 int build_unified(fftso3_plan_s *pl);
@@ -638,6 +644,7 @@ ChunkArgs chunk_args(fftso3_plan_s *pl, Class *c, int i0, int nfc)
     ca.klo = pl->klo;
     ca.ngrp = pl->domain == FFTSO3_DOMAIN_S2 ? c->B : c->B * c->B;
     ca.gil = ws_gil(pl, c);
+    ca.hsplit = c->prfft ? 1 : 0;
     return ca;
 }
 
@@ -971,6 +978,32 @@ int launch_r16(int R, bool fwd, bool real, const FftArgs<float> &a, hipStream_t 
     }
 }
 
+// The two passes of fft_lines.hpp for one bandlimit N (FP64 128, FP32 512)
+template <typename T, int N>
+int launch_prfft(const FftArgs<T> &a, bool fwd, bool real, hipStream_t s)
+{
+    const int nfc = a.ca.nfc, nfb = (nfc + PrCfg<T, N>::F - 1) / PrCfg<T, N>::F;
+    const int grows = xcd_grid(nfb * N * (N / 2));
+    constexpr size_t lrows = pr_rows_lds<T, N>();
+#define PR_COLS(REAL, FWD)                                                                                        \
+    hipLaunchKernelGGL((so3_pr_cols<T, N, REAL, FWD>), dim3(xcd_grid(nfc * N * (N / PrColsCfg<T, N, REAL>::CB))), \
+                       dim3(PrColsCfg<T, N, REAL>::NTHR), (pr_cols_lds<T, N, REAL>()), s, a)
+    if (fwd) {
+        if (real) PR_COLS(true, true);
+        else PR_COLS(false, true);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((so3_pr_rows<T, N, true>), dim3(grows), dim3(PrCfg<T, N>::NTHR), lrows, s, a);
+    } else {
+        hipLaunchKernelGGL((so3_pr_rows<T, N, false>), dim3(grows), dim3(PrCfg<T, N>::NTHR), lrows, s, a);
+        HIP_TRY(hipGetLastError());
+        if (real) PR_COLS(true, false);
+        else PR_COLS(false, false);
+    }
+#undef PR_COLS
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
 // The alpha/gamma FFTs of one pass: grid -> workspace (forward) or workspace -> grid (inverse)
 template <typename T>
 int fft_stage(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst, hipStream_t s)
@@ -1014,6 +1047,9 @@ int fft_stage(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const 
             size_t lds = (size_t)N * N * sizeof(T2) + tables;
             if (real) hipLaunchKernelGGL((so3_fft_fwd_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
             else hipLaunchKernelGGL((so3_fft_fwd_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+        } else if (c->prfft) {
+            if constexpr (sizeof(T) == 8) return launch_prfft<T, 128>(a, true, real, s);
+            else return launch_prfft<T, 512>(a, true, real, s);
         } else {
             a.tile = c->so3_tile;
             int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
@@ -1060,6 +1096,9 @@ int fft_stage(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const 
         size_t lds = (size_t)N * N * sizeof(T2) + tables;
         if (real) hipLaunchKernelGGL((so3_fft_inv_slice<T, true>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
         else hipLaunchKernelGGL((so3_fft_inv_slice<T, false>), dim3(xcd_grid(nfc * N)), dim3(fused_threads(N)), lds, s, a);
+    } else if (c->prfft) {
+        if constexpr (sizeof(T) == 8) return launch_prfft<T, 128>(a, false, real, s);
+        else return launch_prfft<T, 512>(a, false, real, s);
     } else {
         a.tile = c->so3_tile;
         int nunit = nfc * N * ((N + a.tile - 1) / a.tile);
@@ -1570,6 +1609,12 @@ int set_kernel_attrs()
     ALLOW_LDS((so3_fft128_fwd<false>));
     ALLOW_LDS((so3_fft128_inv<true>));
     ALLOW_LDS((so3_fft128_inv<false>));
+    ALLOW_LDS((so3_pr_cols<float, 512, false, true>));
+    ALLOW_LDS((so3_pr_cols<float, 512, false, false>));
+    ALLOW_LDS((so3_pr_cols<float, 512, true, true>));
+    ALLOW_LDS((so3_pr_cols<float, 512, true, false>));
+    ALLOW_LDS((so3_pr_rows<float, 512, true>));
+    ALLOW_LDS((so3_pr_rows<float, 512, false>));
     ALLOW_LDS(contract_fwd_mx64);
     ALLOW_LDS(contract_inv_mx64);
     ALLOW_LDS((so3_fft128_fwd2<true>));

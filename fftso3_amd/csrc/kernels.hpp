@@ -54,7 +54,22 @@ struct ChunkArgs {
     int klo;                // blocked rows: [k >> klo][g][k & (2^klo - 1)]
     int ngrp;               // (m,n) groups of the bandlimit: B^2 (SO(3)) or B (S^2)
     int gil;                // log2 of the groups whose records interleave per function (ws_rec)
+    int hsplit;             // SO(3) rows [half][function][2 pairs] (fft_lines.hpp) instead of [function][4 pairs]
 };
+
+// column c of a chunk's workspace row (real values) -> function, pair, re/im
+__device__ __forceinline__ void ws_col(const ChunkArgs &ca, int NP, int c, int &fi, int &p, int &ri)
+{
+    ri = c & 1;
+    if (ca.hsplit) {
+        const int h = c / (4 * ca.C), q = c - h * 4 * ca.C;
+        fi = q >> 2;
+        p = 2 * h + ((q >> 1) & 1);
+    } else {
+        p = (c >> 1) % NP;
+        fi = c / (2 * NP);
+    }
+}
 
 // Workspace row of (group g, beta index k); a row holds C functions x NP pairs of complex values.
 // The blocked order keeps the rows one FFT slice (one k) touches within 2^klo rows of stride of

@@ -1166,7 +1166,7 @@ __global__ void __launch_bounds__(256) contract_fwd_tab(ConArgs<float> a)
     }
     __syncthreads();
     const float *tab = reinterpret_cast<const float *>(a.dtab) + a.dtab_off[g];
-    const int nct = NCOL > 32 ? 2 : 1;
+    const int nct = (ca.hsplit ? CW : NCOL) > 32 ? 2 : 1;   // column tiles holding functions of the chunk
     // this lane's output column: function fi, pair p, re/im
     const int c0 = cl, c1 = 32 + cl;
     const float *Sb0 = Sl + kh * H * TABF_COLS + ((c0 + 32 * kh) & (TABF_COLS - 1));
@@ -1220,9 +1220,10 @@ __global__ void __launch_bounds__(256) contract_fwd_tab(ConArgs<float> a)
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int c = h ? c1 : c0;
-            if (c >= NCOL || (h && nct < 2)) continue;
-            const int ri = c & 1, p = (c >> 1) % NP, fi = c / (2 * NP);
-            if (!mx_pair_valid<NP>(m0, n0, p)) continue;
+            if (c >= CW || (h && nct < 2)) continue;
+            int fi, p, ri;
+            ws_col(ca, NP, c, fi, p, ri);
+            if (fi >= ca.nfc || !mx_pair_valid<NP>(m0, n0, p)) continue;
             const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
             const MxPair<NP> pr(m0, n0, p);
             float *out = dst + (a.pt ? 2L * fi : 2L * ca.coef_off[ca.fn[fi]]) + ri;
@@ -1258,7 +1259,7 @@ __global__ void __launch_bounds__(256) contract_fwd_tab2(ConArgs<float> a)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     const float *tab = reinterpret_cast<const float *>(a.dtab) + a.dtab_off[g];
-    const int nct = NCOL > 32 ? 2 : 1;
+    const int nct = (ca.hsplit ? CW : NCOL) > 32 ? 2 : 1;   // column tiles holding functions of the chunk
     const int c0 = cl, c1 = 32 + cl;
     const float *Sb0 = Sl + 4 * kh * TABF_COLS + ((c0 + 32 * kh) & (TABF_COLS - 1));
     const float *Sb1 = Sl + 4 * kh * TABF_COLS + ((c1 + 32 * kh) & (TABF_COLS - 1));
@@ -1335,9 +1336,10 @@ __global__ void __launch_bounds__(256) contract_fwd_tab2(ConArgs<float> a)
 #pragma unroll
         for (int hc = 0; hc < 2; hc++) {
             const int c = hc ? c1 : c0;
-            if (c >= NCOL || (hc && nct < 2)) continue;
-            const int ri = c & 1, p = (c >> 1) % NP, fi = c / (2 * NP);
-            if (!mx_pair_valid<NP>(m0, n0, p)) continue;
+            if (c >= CW || (hc && nct < 2)) continue;
+            int fi, p, ri;
+            ws_col(ca, NP, c, fi, p, ri);
+            if (fi >= ca.nfc || !mx_pair_valid<NP>(m0, n0, p)) continue;
             const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
             const MxPair<NP> pr(m0, n0, p);
             float *out = dst + (a.pt ? 2L * fi : 2L * ca.coef_off[ca.fn[fi]]) + ri;
@@ -1373,13 +1375,14 @@ __global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
     for (int e = tid; e < M4 * 64; e += 256) {
         const int lr = e >> 6, c = e & 63;
         float v = 0.f;
-        if (lr < M && c < NCOL) {
-            const int p = (c >> 1) % NP, fi = c / (2 * NP);
+        int fi, p, ri;
+        ws_col(ca, NP, c, fi, p, ri);
+        if (lr < M && c < CW && fi < ca.nfc) {
             if (mx_pair_valid<NP>(m0, n0, p)) {
                 const long idx = MxPair<NP>(m0, n0, p).packed(m0 + lr);
                 const float2 f = a.pt ? Fc[idx * C + fi] : Fc[ca.coef_off[ca.fn[fi]] + idx];
                 const float cf = NP == 2 ? 1.f : (float)(2 * (m0 + lr) + 1);   // SO(3): (2l+1)
-                v = (float)mx_pair_sign<NP>(m0, n0, p) * cf * ((c & 1) ? f.y : f.x);
+                v = (float)mx_pair_sign<NP>(m0, n0, p) * cf * (ri ? f.y : f.x);
             }
         }
         Cb[lr * TAB_CPITCH + c] = v;
@@ -1388,7 +1391,7 @@ __global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
     const float *tab = reinterpret_cast<const float *>(a.dtab) + a.dtab_off[g];
     const int k = k0 + 32 * wave + cl;           // this lane's A row (beta index)
     const int kc = k < N ? k : N - 1;            // clamped: rows k >= N are computed, never stored
-    const int nct = NCOL > 32 ? 2 : 1;
+    const int nct = (ca.hsplit ? CW : NCOL) > 32 ? 2 : 1;   // column tiles holding functions of the chunk
     f32x16m_t acc0, acc1;
 #pragma unroll
     for (int j = 0; j < 16; j++) { acc0[j] = 0.f; acc1[j] = 0.f; }
@@ -1427,8 +1430,8 @@ __global__ void __launch_bounds__(256) contract_inv_tab(ConArgs<float> a)
         const int kk = k0 + 32 * wave + 8 * (j >> 2) + 4 * kh + (j & 3);
         if (kk < N) {
             float *row = G + ws_row(ca, g, kk) * CW;
-            if (cl < NCOL) row[cl] = acc0[j];
-            if (nct > 1 && 32 + cl < NCOL) row[32 + cl] = acc1[j];
+            if (cl < CW) row[cl] = acc0[j];
+            if (nct > 1 && 32 + cl < CW) row[32 + cl] = acc1[j];
         }
     }
 }

@@ -117,8 +117,11 @@ __global__ void __launch_bounds__(MX64_THREADS) contract_fwd_mx64(ConArgs<double
     }
     const int ct = wave & 3, rt0 = wave >> 2;   // column tile; row tiles rt0, rt0 + 2
     const int rq = lane >> 4, cq = lane & 15;
-    // epilogue constants: column 16 ct + cq of a block is (function 8 fb + fl, pair p, re/im ri)
-    const int col = 16 * ct + cq, ri = col & 1, p = (col >> 1) & 3, fl = col >> 3;
+    // epilogue constants: column 16 ct + cq of a block is (function 8 fb + fl, pair p, re/im ri);
+    // rows [function][4 pairs], or [half][function][2 pairs] (half records, fft_lines.hpp)
+    const int col = 16 * ct + cq, ri = col & 1;
+    const int p = ca.hsplit ? 2 * (col >> 5) + ((col >> 1) & 1) : (col >> 1) & 3;
+    const int fl = ca.hsplit ? (col >> 2) & 7 : col >> 3;
     const bool pvalid = so3_pair_valid(m0, n0, p);
     int m, n;
     so3_pair(m0, n0, p, m, n);
@@ -206,7 +209,9 @@ __global__ void __launch_bounds__(MX64_THREADS) contract_inv_mx64(ConArgs<double
             const int e = tid + i * MX64_THREADS;
             double2 v = make_double2(0.0, 0.0);
             if (e < total) {
-                const int lr = e / NCC, cc = e - lr * NCC, p = cc & 3, fi = MX64_FB * fb + (cc >> 2);
+                const int lr = e / NCC, cc = e - lr * NCC;
+                const int p = ca.hsplit ? 2 * (cc >> 4) + (cc & 1) : cc & 3;
+                const int fi = MX64_FB * fb + (ca.hsplit ? (cc >> 1) & 7 : cc >> 2);
                 if (lr < M && fi < ca.nfc) {
                     int m, n;
                     so3_pair(m0, n0, p, m, n);
@@ -273,7 +278,7 @@ __global__ void __launch_bounds__(MX64_THREADS) contract_inv_mx64(ConArgs<double
             }
         }
         // G[k][block columns]: 16 lanes write 128 contiguous bytes of a beta row of the block
-        const bool live = MX64_FB * fb + (col >> 3) < ca.nfc;
+        const bool live = MX64_FB * fb + (ca.hsplit ? (col >> 2) & 7 : col >> 3) < ca.nfc;
 #pragma unroll
         for (int t = 0; t < 2; t++) {
             const int rt = rt0 + 2 * t;
