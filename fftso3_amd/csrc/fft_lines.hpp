@@ -17,13 +17,14 @@
 // The 1-D DFTs are in-LDS Stockham passes (radix 8, then 4 or 2 for what remains), unnormalised
 // forward DFTs X[f] = sum_j x[j] W_N^(f j), as every FFT of this library; element e of a line
 // sits at e + e / 8 (one pad per 8 elements keeps the strided accesses of the early passes off
-// a few banks).
+// a few banks), and lines are N + N/8 + 1 elements apart, so that lanes reading the same element
+// of consecutive lines (pass A's column staging, pass B's records) fall in different banks.
 #pragma once
 
 #include "kernels.hpp"
 
 __host__ __device__ constexpr int pr_pad(int e) { return e + (e >> 3); }
-__host__ __device__ constexpr int pr_pitch(int N) { return N + N / 8; }
+__host__ __device__ constexpr int pr_pitch(int N) { return N + N / 8 + 1; }
 
 // one Stockham pass of radix R over L lines of N points (Ns: the length of the sub-transforms
 // already done); PER butterflies per thread at most
