@@ -7,6 +7,7 @@
 #include "multi.hpp"
 #include "mfma64.hpp"
 #include "fft_lines.hpp"
+#include "mfma_otf.hpp"
 
 #include "../../include/fftso3_amd.h"
 
@@ -127,6 +128,7 @@ struct Class {
     DevBuf d_items_fwd, d_items_inv;
     bool mx64 = false;                       // batch-wide FP64 MFMA contractions (mfma64.hpp)
     bool prfft = false;                      // two-pass FFTs of fft_lines.hpp (workspace in half records)
+    bool otf = false;                        // split-BF16 contractions of mfma_otf.hpp (l_max = 255, chunks of 8)
     DevBuf d_items_fwd64, d_items_inv64;
     int nitems_fwd64 = 0, nitems_inv64 = 0, kt64 = 0;
     int nitems_fwd = 0, nitems_inv = 0, lt_fwd = 0, kt_inv = 0;
@@ -375,10 +377,14 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     }
     c->mx64 = mx64_eligible(pl, c);
     c->chunk = c->mx || c->mx64 ? (int)c->fns.size() : pl->C;
+    // split precisions at l_max = 255 (complex SO(3), chunks of 8): on-the-fly recursion with
+    // MFMA products (mfma_otf.hpp), over half-record workspace rows and PT coefficients
+    c->otf = (pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6) && pl->domain == FFTSO3_DOMAIN_SO3 &&
+             pl->kind == FFTSO3_COMPLEX && N == OTF_N && c->chunk == 8 && !c->mx && !pl->force_chunked;
     // large bandlimits on the chunked kernels: coefficients go through PT[e][fi] so the contraction
     // moves whole rows of the chunk's functions instead of scattered single values
-    c->pt = !c->mx && pl->kind == FFTSO3_COMPLEX && B > 64 && pl->prec != FFTSO3_BF16X3 &&
-            pl->prec != FFTSO3_BF16X6;
+    c->pt = (!c->mx && pl->kind == FFTSO3_COMPLEX && B > 64 && pl->prec != FFTSO3_BF16X3 &&
+             pl->prec != FFTSO3_BF16X6) || c->otf;
     // Above l_max = 127 the chunked contraction would spend its time in the sequential recursion
     // (one lane per k, the rest of the workgroup idle): the plan computes d once, at full
     // occupancy, into a table it keeps ([g][l - m0][k] in the storage type; 11.5 GB at l_max = 255
@@ -493,7 +499,7 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     // slices larger than LDS whose contraction reads half records: FP64 l_max = 63 (batch-wide MFMA)
     // and FP32 l_max = 255 (d table)
     c->prfft = !s2 && ((pl->prec == FFTSO3_FP64 && c->mx64 && N == 128) ||
-                       (pl->prec == FFTSO3_FP32 && !c->mx && c->d_dtab.p && N == 512));
+                       (pl->prec == FFTSO3_FP32 && !c->mx && c->d_dtab.p && N == 512) || c->otf);
     return FFTSO3_OK;
 }
 int build_unified(fftso3_plan_s *pl);
@@ -940,6 +946,19 @@ int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const v
             a.src = src;
             a.dst = dst;
             const bool x6 = pl->prec == FFTSO3_BF16X6;
+            if (c->otf) {
+                a.pt = 1;
+                const int grid = deal_grid(ca.ngrp, 1);
+                if (fwd) {
+                    if (x6) hipLaunchKernelGGL((contract_fwd_otf<3>), dim3(grid), dim3(OTF_THREADS), otf_fwd_lds_bytes(3, c->B), s, a);
+                    else hipLaunchKernelGGL((contract_fwd_otf<2>), dim3(grid), dim3(OTF_THREADS), otf_fwd_lds_bytes(2, c->B), s, a);
+                } else {
+                    if (x6) hipLaunchKernelGGL((contract_inv_otf<3>), dim3(grid), dim3(OTF_THREADS), otf_inv_lds_bytes(3, c->B), s, a);
+                    else hipLaunchKernelGGL((contract_inv_otf<2>), dim3(grid), dim3(OTF_THREADS), otf_inv_lds_bytes(2, c->B), s, a);
+                }
+                HIP_TRY(hipGetLastError());
+                return FFTSO3_OK;
+            }
             if (a.s2) return x6 ? launch_contract_bf16<2, 3>(c, a, fwd, s) : launch_contract_bf16<2, 2>(c, a, fwd, s);
             return x6 ? launch_contract_bf16<4, 3>(c, a, fwd, s) : launch_contract_bf16<4, 2>(c, a, fwd, s);
         }
@@ -1629,6 +1648,10 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv_bf16<4, 3>));
     ALLOW_LDS((contract_inv_bf16<2, 2>));
     ALLOW_LDS((contract_inv_bf16<2, 3>));
+    ALLOW_LDS((contract_fwd_otf<2>));
+    ALLOW_LDS((contract_fwd_otf<3>));
+    ALLOW_LDS((contract_inv_otf<2>));
+    ALLOW_LDS((contract_inv_otf<3>));
     // s2_rotate also has static LDS: its dynamic part is at most the 128 KB accumulator
     HIP_TRY(hipFuncSetAttribute((const void *)(s2_rotate<true>), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
     HIP_TRY(hipFuncSetAttribute((const void *)(s2_rotate<false>), hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
@@ -1790,10 +1813,10 @@ int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len)
         const char *path = c->mxk ? "batch-wide mfma (k-chunked)" : c->mx ? "batch-wide mfma" : "chunked";
         char line[256];
         ChunkArgs ca = chunk_args<float>(plan, const_cast<Class *>(c), 0, (int)c->fns.size());
-        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s%s\n", c->B - 1,
+        std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s%s%s\n", c->B - 1,
                       (int)c->fns.size(), prec, path, c->chunk, c->mx ? plan->klo : 0, ca.gil, plan->split_mul,
                       c->r16 ? " fft16R" : "", plan->unified ? " unified" : "", c->d_dtab.p ? " d-table" : "",
-                      c->mx64 ? " mfma64" : "", plan->graph ? " graph" : "");
+                      c->mx64 ? " mfma64" : "", c->otf ? " d-on-the-fly" : "", plan->graph ? " graph" : "");
         out += line;
     }
     std::snprintf(buf, (size_t)len, "%s", out.c_str());

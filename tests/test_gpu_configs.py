@@ -49,16 +49,19 @@ def test_uneven_real_fp32_batch(fftso3, torch_cuda, oracle):
         assert rel_l2(gi, oracle.so3_real_inverse(B, coefs[i]).reshape(-1)) < 2e-6, B
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-5), ("bf16x6", 1e-5), ("bf16x3", 5e-5)])
 def test_so3_b256(fftso3, torch_cuda, oracle, prec, tol):
     """Config 5a: l_max=255 (512^3 grids), all 8 functions of the seed-4 workload.  Every function
     round trips on the GPU; the grids of the first and the last function at 12 sampled points each
-    against the oracle's pointwise inverse."""
+    against the oracle's pointwise inverse.  The split precisions run the on-the-fly recursion
+    kernels (d-on-the-fly), FP32 the Wigner-d table."""
     torch = torch_cuda
     B, n = 256, 8
     F = inputs.workload_b256(n)
     print("b256 %s: plan" % prec, flush=True)
     plan = fftso3.Plan(B - 1, n, "so3", "complex", prec)
+    if prec.startswith("bf16"):
+        assert "d-on-the-fly" in plan.describe()
     gdt, cdt = plan.dtypes()
     dF = torch.from_numpy(F.reshape(-1)).to(cdt).cuda()
     g = plan.inverse(dF)
@@ -79,6 +82,30 @@ def test_so3_b256(fftso3, torch_cuda, oracle, prec, tol):
         ref = np.array([oracle.so3_inverse_point(B, F[i], j1 * np.pi / B, (2 * k + 1) * np.pi / (4 * B),
                                                  j2 * np.pi / B) for k, j1, j2 in pts])
         assert rel_l2(got, ref) < tol, i
+
+
+@pytest.mark.parametrize("prec,tol", [("bf16x6", 1e-5), ("bf16x3", 5e-5)])
+def test_so3_b256_split_partial_chunk(fftso3, torch_cuda, prec, tol):
+    """l_max=255 in split precision with 3 functions (a chunk of 8 with 5 empty column slots): the
+    inverse grids against the FP64 GPU path's over every grid point, and the forward of the FP64
+    grids against the coefficients (the FP64 path itself is held to the oracle in test_so3_b256)."""
+    torch = torch_cuda
+    B, n = 256, 3
+    F = inputs.so3_coefficients(B, n, 4256, scale=lambda l: 1.0 / (1.0 + l))
+    ref = fftso3.Plan(B - 1, n, "so3", "complex", "fp64")
+    dF64 = torch.from_numpy(F.reshape(-1)).to(torch.complex128).cuda()
+    g64 = ref.inverse(dF64)
+    plan = fftso3.Plan(B - 1, n, "so3", "complex", prec)
+    assert "d-on-the-fly" in plan.describe()
+    g = plan.inverse(dF64.to(torch.complex64))
+    c = plan.forward(g64.to(torch.complex64))
+    torch.cuda.synchronize()
+    eg = (torch.linalg.norm((g.to(torch.complex128) - g64).reshape(n, -1), dim=1) /
+          torch.linalg.norm(g64.reshape(n, -1), dim=1)).cpu().numpy()
+    ec = (torch.linalg.norm((c.to(torch.complex128) - dF64).reshape(n, -1), dim=1) /
+          torch.linalg.norm(dF64.reshape(n, -1), dim=1)).cpu().numpy()
+    print("b256 %s partial chunk: grid %s coef %s" % (prec, eg, ec), flush=True)
+    assert float(eg.max()) < tol and float(ec.max()) < tol
 
 
 def test_s2_b128_batch4096_fp32(fftso3, torch_cuda, oracle):
