@@ -948,14 +948,18 @@ int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const v
             const bool x6 = pl->prec == FFTSO3_BF16X6;
             if (c->otf) {
                 a.pt = 1;
-                const int grid = deal_grid(ca.ngrp, 1);
+                // one workgroup per group, largest first (a persistent form that claimed groups in a
+                // loop was twice as slow: each group's first load waits also for the previous group's
+                // 128 KB of workspace stores, which a new workgroup does not inherit)
+#define OTF_LAUNCH(K, T, L) hipLaunchKernelGGL((K), dim3(deal_grid(ca.ngrp, 1)), dim3(T), L, s, a)
                 if (fwd) {
-                    if (x6) hipLaunchKernelGGL((contract_fwd_otf<3>), dim3(grid), dim3(OTF_THREADS), otf_fwd_lds_bytes(3, c->B), s, a);
-                    else hipLaunchKernelGGL((contract_fwd_otf<2>), dim3(grid), dim3(OTF_THREADS), otf_fwd_lds_bytes(2, c->B), s, a);
+                    if (x6) OTF_LAUNCH(contract_fwd_otf<3>, OTFF_THREADS, otf_fwd_lds_bytes(3, c->B));
+                    else OTF_LAUNCH(contract_fwd_otf<2>, OTFF_THREADS, otf_fwd_lds_bytes(2, c->B));
                 } else {
-                    if (x6) hipLaunchKernelGGL((contract_inv_otf<3>), dim3(grid), dim3(OTF_THREADS), otf_inv_lds_bytes(3, c->B), s, a);
-                    else hipLaunchKernelGGL((contract_inv_otf<2>), dim3(grid), dim3(OTF_THREADS), otf_inv_lds_bytes(2, c->B), s, a);
+                    if (x6) OTF_LAUNCH(contract_inv_otf<3>, OTFI_THREADS, otf_inv_lds_bytes(3, c->B));
+                    else OTF_LAUNCH(contract_inv_otf<2>, OTFI_THREADS, otf_inv_lds_bytes(2, c->B));
                 }
+#undef OTF_LAUNCH
                 HIP_TRY(hipGetLastError());
                 return FFTSO3_OK;
             }
