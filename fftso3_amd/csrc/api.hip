@@ -141,6 +141,9 @@ struct Class {
     int r16 = 0;                    // R of the N = 16R slice FFTs (fft_r16.hpp) of this class, or 0
     size_t ws_off = 0, fc_off = 0, tmp_off = 0;   // the class's region of the plan's buffers (unified plans)
     bool mxk = false;               // ... with the K-chunked kernels (128 < N, B <= 128)
+    bool mxks = false;              // ... split-BF16 K-chunked kernels (128 < N, B <= 128)
+    DevBuf d_items_mxs_f, d_items_mxs_i;   // their items: (group, degree range), (group, beta range)
+    int nitems_mxs_f = 0, nitems_mxs_i = 0;
     bool pt = false;                // chunked contraction reads / writes transposed coefficients PT
     DevBuf d_dtab, d_dtab_off;      // precomputed d table of the chunked path at large B (or empty)
     int chunk = 8;                  // functions per pipeline pass for this class
@@ -373,7 +376,11 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
                           mxk_lds_bytes(N, B, np, false) + rcb <= (size_t)kMaxLds;
         const bool split = pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6;
         c->mxk = pl->prec == FFTSO3_FP32 && N > 128 && fits && !pl->force_chunked;
-        c->mx = ((pl->prec == FFTSO3_FP32 || split) && N <= 128 && !pl->force_chunked) || c->mxk;
+        const int npc = pl->prec == FFTSO3_BF16X6 ? 3 : 2;
+        c->mxks = split && N > 128 && B <= MXK_MAXROWS && !pl->force_chunked &&
+                  mxks_fwd_lds_bytes(N, npc, np) + rcb <= (size_t)kMaxLds &&
+                  mxks_inv_lds_bytes(npc, np) + rcb <= (size_t)kMaxLds;
+        c->mx = ((pl->prec == FFTSO3_FP32 || split) && N <= 128 && !pl->force_chunked) || c->mxk || c->mxks;
     }
     c->mx64 = mx64_eligible(pl, c);
     c->chunk = c->mx || c->mx64 ? (int)c->fns.size() : pl->C;
@@ -475,6 +482,21 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     for (int g = 0; g < ngrp; g++) mx[g] = g;
     c->nitems_mx = ngrp;
     if ((st = c->d_items_mx.upload(mx))) return st;
+    if (c->mxks) {
+        // split K-chunked kernels: forward items 2 g + h (degrees m0 + 64 h ..), inverse items
+        // g * ceil(N / 128) + kr (beta values 128 kr ..), groups largest first
+        std::vector<int> f, i;
+        const int nkr = (N + MXKS_KR - 1) / MXKS_KR;
+        for (int g = 0; g < ngrp; g++) {
+            int m0 = s2 ? g : (int)std::sqrt((double)g + 0.5);
+            if (!s2) { while (m0 * m0 > g) m0--; while ((m0 + 1) * (m0 + 1) <= g) m0++; }
+            for (int h = 0; h < 2 && m0 + MXKS_LR * h < B; h++) f.push_back(2 * g + h);
+            for (int kr = 0; kr < nkr; kr++) i.push_back(g * nkr + kr);
+        }
+        c->nitems_mxs_f = (int)f.size();
+        c->nitems_mxs_i = (int)i.size();
+        if ((st = c->d_items_mxs_f.upload(f)) || (st = c->d_items_mxs_i.upload(i))) return st;
+    }
     const int CC = c->chunk;
     std::vector<int> ident(CC);
     for (int i = 0; i < CC; i++) ident[i] = i;
@@ -808,13 +830,31 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             StageTimer t(pl, s, 1);
             a.src = pl->d_ws.p;
             a.dst = pl->d_fc.p;
-            if (pl->prec == FFTSO3_BF16X6)
+            if (c->mxks) {
+                MxArgs af = a;
+                af.items = (const int *)c->d_items_mxs_f.p;
+                af.nitems = c->nitems_mxs_f;
+                const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+                af.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + af.nitems - 1) / af.nitems)));
+                const int fgrid = deal_grid(af.nitems * af.nsplit, af.nsplit);
+                if (pl->prec == FFTSO3_BF16X6)
+                    hipLaunchKernelGGL((contract_fwd_mxks<NP, 3>), dim3(fgrid), dim3(512),
+                                       mxks_fwd_lds_bytes(c->N, 3, NP) + rcb, s, af);
+                else
+                    hipLaunchKernelGGL((contract_fwd_mxks<NP, 2>), dim3(fgrid), dim3(512),
+                                       mxks_fwd_lds_bytes(c->N, 2, NP) + rcb, s, af);
+            } else if (pl->prec == FFTSO3_BF16X6)
                 hipLaunchKernelGGL((contract_fwd_mxs<NP, 3>), dim3(mgrid), dim3(256), mxs_fwd_lds_bytes(3) + rcb, s, a);
             else if (pl->prec == FFTSO3_BF16X3)
                 hipLaunchKernelGGL((contract_fwd_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_fwd_lds_bytes(2) + rcb, s, a);
-            else if (c->mxk)
-                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(mgrid), dim3(256),
-                                   mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a);
+            else if (c->mxk) {
+                // 8 waves per workgroup over blocks of 256 columns: half as many units
+                MxArgs a8 = a;
+                const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+                a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
+                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
+                                   dim3(512), mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a8);
+            }
             else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2) {
                 // the 2^gil groups that share workspace lines in one workgroup
                 MxArgs ag = a;
@@ -869,7 +909,18 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         StageTimer t(pl, s, 4);
         a.src = pl->d_fc.p;
         a.dst = pl->d_ws.p;
-        if (pl->prec == FFTSO3_BF16X6)
+        if (c->mxks) {
+            MxArgs ai = a;
+            ai.items = (const int *)c->d_items_mxs_i.p;
+            ai.nitems = c->nitems_mxs_i;
+            const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+            ai.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + ai.nitems - 1) / ai.nitems)));
+            const int igrid = deal_grid(ai.nitems * ai.nsplit, ai.nsplit);
+            if (pl->prec == FFTSO3_BF16X6)
+                hipLaunchKernelGGL((contract_inv_mxks<NP, 3>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(3, NP) + rcb, s, ai);
+            else
+                hipLaunchKernelGGL((contract_inv_mxks<NP, 2>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(2, NP) + rcb, s, ai);
+        } else if (pl->prec == FFTSO3_BF16X6)
             hipLaunchKernelGGL((contract_inv_mxs<NP, 3>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(3) + rcb, s, a);
         else if (pl->prec == FFTSO3_BF16X3)
             hipLaunchKernelGGL((contract_inv_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(2) + rcb, s, a);
@@ -1249,7 +1300,7 @@ bool unified_eligible(const fftso3_plan_s *pl)
         pl->klo != 0 || pl->classes.size() < 2 || pl->classes.size() > MULTI_MAX_FFT)
         return false;
     for (const Class *c : pl->classes) {
-        if (!c->mx || c->mxk) return false;
+        if (!c->mx || c->mxk || c->mxks) return false;
         if (c->N == 128) {
             if (ws_gil(pl, c) != 1) return false;
         } else if (!c->r16) {
@@ -1627,6 +1678,14 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mxk<2>));
     ALLOW_LDS((contract_inv_mxk<4, 8>));
     ALLOW_LDS((contract_inv_mxk<2, 8>));
+    ALLOW_LDS((contract_fwd_mxks<4, 2>));
+    ALLOW_LDS((contract_fwd_mxks<4, 3>));
+    ALLOW_LDS((contract_fwd_mxks<2, 2>));
+    ALLOW_LDS((contract_fwd_mxks<2, 3>));
+    ALLOW_LDS((contract_inv_mxks<4, 2>));
+    ALLOW_LDS((contract_inv_mxks<4, 3>));
+    ALLOW_LDS((contract_inv_mxks<2, 2>));
+    ALLOW_LDS((contract_inv_mxks<2, 3>));
     ALLOW_LDS((contract_inv_mx32<2>));
     ALLOW_LDS((so3_fft128_fwd<true>));
     ALLOW_LDS((so3_fft128_fwd<false>));
@@ -1814,7 +1873,7 @@ int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len)
     const char *prec = plan->prec == FFTSO3_FP64 ? "fp64" : plan->prec == FFTSO3_FP32 ? "fp32"
                      : plan->prec == FFTSO3_BF16X6 ? "bf16x6" : "bf16x3";
     for (const Class *c : plan->classes) {
-        const char *path = c->mxk ? "batch-wide mfma (k-chunked)" : c->mx ? "batch-wide mfma" : "chunked";
+        const char *path = c->mxk || c->mxks ? "batch-wide mfma (k-chunked)" : c->mx ? "batch-wide mfma" : "chunked";
         char line[256];
         ChunkArgs ca = chunk_args<float>(plan, const_cast<Class *>(c), 0, (int)c->fns.size());
         std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s%s%s\n", c->B - 1,

@@ -20,22 +20,16 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ unsigned short bf16_rne(float x)
-{
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_val(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
-
+// bf16 pieces: the hardware conversion (v_cvt_pk_bf16_f32) rounds to nearest even; each residual
+// is exact
 template <int NPC> __device__ __forceinline__ void split_d(double v, unsigned short *out)
 {
     double r = v;
 #pragma unroll
     for (int i = 0; i < NPC; i++) {
-        unsigned short b = bf16_rne((float)r);
-        out[i] = b;
-        r -= (double)bf16_val(b);
+        const __bf16 b = (__bf16)(float)r;
+        out[i] = __builtin_bit_cast(unsigned short, b);
+        r -= (double)(float)b;
     }
 }
 template <int NPC> __device__ __forceinline__ void split_f(float v, unsigned short *out)
@@ -43,9 +37,9 @@ template <int NPC> __device__ __forceinline__ void split_f(float v, unsigned sho
     float r = v;
 #pragma unroll
     for (int i = 0; i < NPC; i++) {
-        unsigned short b = bf16_rne(r);
-        out[i] = b;
-        r -= bf16_val(b);
+        const __bf16 b = (__bf16)r;
+        out[i] = __builtin_bit_cast(unsigned short, b);
+        r -= (float)b;
     }
 }
 

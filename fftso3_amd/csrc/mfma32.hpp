@@ -890,61 +890,46 @@ __global__ void __launch_bounds__(256) coef_transpose(ChunkArgs ca, int ncoef, c
 // ---------------------------------------------------------------------------------------------
 // Batch-wide contraction for 128 < N with the group's Wigner-d block resident in LDS
 // (32 * ceil(M/32) rows x N columns, pitch N+1: up to M = 128, N = 256 -- S^2 at l_max = 127,
-// SO(3) up to l_max = 127).  The spectrum streams through LDS in chunks of 32 beta rows x 128
-// columns; wave w multiplies block columns [32w, 32w+32) in their workspace order (function,
-// pair, re/im), so its coefficient rows are 64 contiguous bytes per pair.  A lane's pair varies
-// with its column, so the packed index of each (degree, pair) comes from a per-group table in LDS.
+// SO(3) up to l_max = 127).  Eight waves per workgroup; wave w multiplies block columns
+// [32w, 32w+32) of a 256-column block in their workspace order (function, pair, re/im), so its
+// coefficient rows are 64 contiguous bytes per pair.  The forward loads each lane's spectrum
+// operands straight from HBM into registers, eight MFMA steps ahead, with no LDS staging and no
+// barrier in the block loop (an earlier form staged 32-row chunks of the block through LDS with
+// four waves: S^2 l_max = 127 x 4096 forward 1.39-1.53 ms).  A lane's pair varies with its column,
+// so the packed index of each (degree, pair) comes from a per-group table in LDS.
 // ---------------------------------------------------------------------------------------------
-#define MXK_LCHUNK 32
 #define MXK_MAXROWS 128
 
-// forward: D [32 nrt][N32 + 1] + spectrum chunk [32][MX_LPITCH] + index table [NP][MXK_MAXROWS];
-// inverse: D [M4][N + 1] + index table
+// forward: D [32 nrt][N32 + 1] + index table [NP][MXK_MAXROWS]; inverse: D [M4][N + 1] + index table
 __host__ __device__ inline size_t mxk_lds_bytes(int N, int B, int NP, bool fwd)
 {
     const int N32 = (N + 31) & ~31;
     const size_t table = (size_t)NP * MXK_MAXROWS * 4;
-    if (fwd) return ((size_t)32 * ((B + 31) / 32) * (N32 + 1) + (size_t)MXK_LCHUNK * MX_LPITCH) * 4 + table;
+    if (fwd) return (size_t)32 * ((B + 31) / 32) * (N32 + 1) * 4 + table;
     return (size_t)((B + 3) & ~3) * (N + 1) * 4 + table;
 }
 
-template <int NP>
+template <int NP, int NW>
 __device__ __forceinline__ void contract_fwd_mxk_body(const MxArgs &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2, NCOL = ca.nfc * NP * 2;
-    const int nfb = (ca.nfc * NP * 2 + 127) / 128;          // column blocks of 128
+    const int nfb = (NCOL + 32 * NW - 1) / (32 * NW);       // column blocks of 32 NW
     int g, fb0, fb1;
     if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
-    const int N32 = (N + 31) & ~31, NK = N32 / 32;
+    const int N32 = (N + 31) & ~31, NS = N32 / 2;
     const int M = B - m0, nrt = (M + 31) / 32, DP = N32 + 1;
     float *D = reinterpret_cast<float *>(smem_raw);                 // [32 nrt][DP]
-    float *L = D + (size_t)32 * nrt * DP;                           // [32][MX_LPITCH]
-    int *etab = reinterpret_cast<int *>(L + MXK_LCHUNK * MX_LPITCH); // [NP][MXK_MAXROWS]
+    int *etab = reinterpret_cast<int *>(D + (size_t)32 * nrt * DP); // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
-    const float *S = reinterpret_cast<const float *>(a.src);
-    // load role: rows r0 + 8i (i < 4) of a chunk, columns 4q..4q+3 of the block
-    const int q = tid & 31, r0 = tid >> 5;
-    float4 pf0, pf1, pf2, pf3;
-#define MXK_LOAD_CHUNK(fb_, kc_)                                                                    \
-    {                                                                                               \
-        const int c_ = (fb_) * 128 + 4 * q;                                                         \
-        const long co_ = c_ < NCOL ? c_ : 0;                                                        \
-        const int k_ = 32 * (kc_) + r0;                                                             \
-        pf0 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_, N - 1)) * CW + co_);      \
-        pf1 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_ + 8, N - 1)) * CW + co_);  \
-        pf2 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_ + 16, N - 1)) * CW + co_); \
-        pf3 = *reinterpret_cast<const float4 *>(S + ws_row(ca, g, min(k_ + 24, N - 1)) * CW + co_); \
-    }
-    MXK_LOAD_CHUNK(fb0, 0)
     RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, B, NP, true));
-    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 256);
+    group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 64 * NW);
     __syncthreads();
-    for (int k = tid; k < N32; k += 256) {
+    for (int k = tid; k < N32; k += 64 * NW) {
         if (k < N) {
             const double wk = a.rt.w[k];
             group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
@@ -955,45 +940,50 @@ __device__ __forceinline__ void contract_fwd_mxk_body(const MxArgs &a, int unit)
             for (int r = 0; r < 32 * nrt; r++) D[r * DP + k] = 0.f;
         }
     }
-    for (int i = tid; i < NP * MXK_MAXROWS; i += 256) {
+    for (int i = tid; i < NP * MXK_MAXROWS; i += 64 * NW) {
         const int p = i / MXK_MAXROWS, r = i - p * MXK_MAXROWS;
         etab[i] = MxPair<NP>(m0, n0, p).packed(m0 + min(r, M - 1));
     }
-    // output role: block column 32 wave + cl -> (function, pair, re/im)
+    __syncthreads();
+    // block column 32 wave + cl -> (function, pair, re/im); the lane's B operand of step s is the
+    // spectrum at beta 2 s + kh, loaded straight into registers eight steps ahead (rows past N - 1
+    // read row N - 1: finite values that meet zero columns of D)
     const int cb = 32 * wave + cl;
     const int p = (cb >> 1) % NP;
     const float sg = (float)mx_pair_sign<NP>(m0, n0, p);
     const bool pvalid = mx_pair_valid<NP>(m0, n0, p);
+    const float *S = reinterpret_cast<const float *>(a.src);
     float *PT = reinterpret_cast<float *>(a.dst);
+    const float *Dw = D + cl * DP + kh;
     for (int fb = fb0; fb < fb1; fb++) {
+        const int c = fb * 32 * NW + cb, cv = c < NCOL ? c : 0;
+        const float *src = S + cv;
+        auto load = [&](float *x, int s0) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = src[ws_row(ca, g, min(2 * (s0 + u) + kh, N - 1)) * CW];
+        };
         f32x16m_t acc[4];
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int j = 0; j < 16; j++) acc[t][j] = 0.f;
-        for (int kc = 0; kc < NK; kc++) {
-            __syncthreads();                             // D and the table written / chunk consumed
-            *reinterpret_cast<float4 *>(L + r0 * MX_LPITCH + 4 * q) = pf0;
-            *reinterpret_cast<float4 *>(L + (r0 + 8) * MX_LPITCH + 4 * q) = pf1;
-            *reinterpret_cast<float4 *>(L + (r0 + 16) * MX_LPITCH + 4 * q) = pf2;
-            *reinterpret_cast<float4 *>(L + (r0 + 24) * MX_LPITCH + 4 * q) = pf3;
-            __syncthreads();
-            {
-                const int nf = kc + 1 < NK ? fb : fb + 1, nk = kc + 1 < NK ? kc + 1 : 0;
-                if (nf < fb1) MXK_LOAD_CHUNK(nf, nk)  // the next chunk streams in during this one
-            }
-            // rows k >= N of a chunk hold finite copies of row N-1 and meet zero columns of D
-            const float *Lw = L + kh * MX_LPITCH + cb;
-            const float *Dw = D + cl * DP + 32 * kc + kh;
+        float x[8], xn[8];
+        load(x, 0);
+        for (int s0 = 0; s0 < NS; s0 += 8) {
+            if (s0 + 8 < NS) load(xn, s0 + 8);
 #pragma unroll
-            for (int s = 0; s < 16; s++) {
-                const float b = Lw[2 * s * MX_LPITCH];
+            for (int u = 0; u < 8; u++) {
+                if (s0 + u < NS) {
 #pragma unroll
-                for (int t = 0; t < 4; t++)
-                    if (t < nrt) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(Dw[32 * t * DP + 2 * s], b, acc[t], 0, 0, 0);
+                    for (int t = 0; t < 4; t++)
+                        if (t < nrt)
+                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(Dw[32 * t * DP + 2 * (s0 + u)], x[u], acc[t], 0, 0, 0);
+                }
             }
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = xn[u];
         }
-        const int c = fb * 128 + cb, fn = c / (2 * NP);
+        const int fn = c / (2 * NP);
         const bool ok = pvalid && c < NCOL;
         float *o = PT + 2 * fn + (c & 1);
 #pragma unroll
@@ -1011,12 +1001,11 @@ __device__ __forceinline__ void contract_fwd_mxk_body(const MxArgs &a, int unit)
 }
 
 template <int NP>
-__global__ void __launch_bounds__(256) contract_fwd_mxk(MxArgs a)
+__global__ void __launch_bounds__(512) contract_fwd_mxk(MxArgs a)
 {
-    contract_fwd_mxk_body<NP>(a, mx_block_unit(a));
+    contract_fwd_mxk_body<NP, 8>(a, mx_block_unit(a));
 }
 
-#undef MXK_LOAD_CHUNK
 
 // Inverse with the group's c_l d^l block resident in LDS ([M4][N+1], rows l - m0 zero-padded to a
 // multiple of 4); wave w produces block columns [32w, 32w+32) for every beta row.

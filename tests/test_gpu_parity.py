@@ -387,6 +387,39 @@ def test_split_bf16_batchwide(fftso3, torch_cuda, oracle, prec, domain, B, kind)
     plan.close()
 
 
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("domain,B,kind,n", [("s2", 75, "complex", 37), ("s2", 128, "complex", 37), ("s2", 100, "real", 37),
+                                             ("so3", 70, "complex", 19), ("so3", 100, "real", 5)])
+def test_split_bf16_batchwide_large_n(fftso3, torch_cuda, oracle, prec, domain, B, kind, n):
+    """Split-BF16 on the batch-wide K-chunked kernels for 128 < N <= 256 (contract_fwd_mxks /
+    contract_inv_mxks: items of 64 degrees and 128 beta values, N not a multiple of 32 at B = 75
+    and 70, partial column blocks): every function round trips, the first and the last against
+    the oracle.  BF16X6 holds the FP32 bar, BF16X3 its own."""
+    torch = torch_cuda
+    cplx = kind == "complex"
+    if domain == "so3":
+        F = inputs.so3_coefficients(B, n, 7300 + B, complex_=cplx, scale=lambda l: 1.0 / (1.0 + l))
+    else:
+        F = inputs.s2_coefficients(B, n, 7300 + B, complex_=cplx, scale=lambda l: 1.0 / (1.0 + l))
+    plan = fftso3.Plan(B - 1, n, domain, kind, prec)
+    assert "k-chunked" in plan.describe()
+    gdt, cdt = plan.dtypes()
+    g = plan.inverse(_to_dev(torch, F, cdt))
+    c = plan.forward(g)
+    torch.cuda.synchronize()
+    g = g.cpu().numpy().reshape(n, -1)
+    c = c.cpu().numpy().reshape(n, -1)
+    tol = FP32_TOL if prec == "bf16x6" else SPLIT_TOL["bf16x3"]
+    assert rel_l2(c, F) < tol
+    for i in (0, n - 1):
+        if domain == "so3":
+            ref = oracle.so3_inverse(B, F[i]) if cplx else oracle.so3_real_inverse(B, F[i])
+        else:
+            ref = oracle.s2_inverse(B, F[i]) if cplx else oracle.s2_real_inverse(B, F[i])
+        assert rel_l2(g[i], ref) < tol, i
+    plan.close()
+
+
 def test_split_bf16_chunked_kernels(fftso3, torch_cuda, oracle):
     """set_chunk(8) keeps the chunked split-BF16 kernels (mfma.hpp) reachable at l_max <= 63."""
     torch = torch_cuda
