@@ -482,9 +482,9 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     for (int g = 0; g < ngrp; g++) mx[g] = g;
     c->nitems_mx = ngrp;
     if ((st = c->d_items_mx.upload(mx))) return st;
-    if (c->mxks) {
-        // split K-chunked kernels: forward items 2 g + h (degrees m0 + 64 h ..), inverse items
-        // g * ceil(N / 128) + kr (beta values 128 kr ..), groups largest first
+    if (c->mx && (pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6)) {
+        // split batch-wide kernels (mxks): forward items 2 g + h (degrees m0 + 64 h ..), inverse
+        // items g * ceil(N / 128) + kr (beta values 128 kr ..), groups largest first
         std::vector<int> f, i;
         const int nkr = (N + MXKS_KR - 1) / MXKS_KR;
         for (int g = 0; g < ngrp; g++) {
@@ -830,24 +830,20 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             StageTimer t(pl, s, 1);
             a.src = pl->d_ws.p;
             a.dst = pl->d_fc.p;
-            if (c->mxks) {
+            if (pl->prec == FFTSO3_BF16X6 || pl->prec == FFTSO3_BF16X3) {
                 MxArgs af = a;
                 af.items = (const int *)c->d_items_mxs_f.p;
                 af.nitems = c->nitems_mxs_f;
                 const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
                 af.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + af.nitems - 1) / af.nitems)));
-                const int fgrid = deal_grid(af.nitems * af.nsplit, af.nsplit);
+                const int fgrid = deal_grid(af.nitems * af.nsplit, af.nsplit << ca.gil);
                 if (pl->prec == FFTSO3_BF16X6)
                     hipLaunchKernelGGL((contract_fwd_mxks<NP, 3>), dim3(fgrid), dim3(512),
                                        mxks_fwd_lds_bytes(c->N, 3, NP) + rcb, s, af);
                 else
                     hipLaunchKernelGGL((contract_fwd_mxks<NP, 2>), dim3(fgrid), dim3(512),
                                        mxks_fwd_lds_bytes(c->N, 2, NP) + rcb, s, af);
-            } else if (pl->prec == FFTSO3_BF16X6)
-                hipLaunchKernelGGL((contract_fwd_mxs<NP, 3>), dim3(mgrid), dim3(256), mxs_fwd_lds_bytes(3) + rcb, s, a);
-            else if (pl->prec == FFTSO3_BF16X3)
-                hipLaunchKernelGGL((contract_fwd_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_fwd_lds_bytes(2) + rcb, s, a);
-            else if (c->mxk) {
+            } else if (c->mxk) {
                 // 8 waves per workgroup over blocks of 256 columns: half as many units
                 MxArgs a8 = a;
                 const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
@@ -909,22 +905,18 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
         StageTimer t(pl, s, 4);
         a.src = pl->d_fc.p;
         a.dst = pl->d_ws.p;
-        if (c->mxks) {
+        if (pl->prec == FFTSO3_BF16X6 || pl->prec == FFTSO3_BF16X3) {
             MxArgs ai = a;
             ai.items = (const int *)c->d_items_mxs_i.p;
             ai.nitems = c->nitems_mxs_i;
             const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
             ai.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + ai.nitems - 1) / ai.nitems)));
-            const int igrid = deal_grid(ai.nitems * ai.nsplit, ai.nsplit);
+            const int igrid = deal_grid(ai.nitems * ai.nsplit, ai.nsplit << ca.gil);
             if (pl->prec == FFTSO3_BF16X6)
                 hipLaunchKernelGGL((contract_inv_mxks<NP, 3>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(3, NP) + rcb, s, ai);
             else
                 hipLaunchKernelGGL((contract_inv_mxks<NP, 2>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(2, NP) + rcb, s, ai);
-        } else if (pl->prec == FFTSO3_BF16X6)
-            hipLaunchKernelGGL((contract_inv_mxs<NP, 3>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(3) + rcb, s, a);
-        else if (pl->prec == FFTSO3_BF16X3)
-            hipLaunchKernelGGL((contract_inv_mxs<NP, 2>), dim3(mgrid), dim3(256), mxs_inv_lds_bytes(2) + rcb, s, a);
-        else if (c->mxk)
+        } else if (c->mxk)
         {
             // 8 waves per workgroup over blocks of 256 columns: half as many units
             MxArgs a8 = a;
@@ -1742,14 +1734,6 @@ int set_kernel_attrs()
     ALLOW_LDS_Q(contract_fwd_multi_s<2>);
     ALLOW_LDS_Q(contract_inv_multi_s<2>);
 #undef ALLOW_LDS_Q
-    ALLOW_LDS((contract_fwd_mxs<4, 2>));
-    ALLOW_LDS((contract_fwd_mxs<4, 3>));
-    ALLOW_LDS((contract_fwd_mxs<2, 2>));
-    ALLOW_LDS((contract_fwd_mxs<2, 3>));
-    ALLOW_LDS((contract_inv_mxs<4, 2>));
-    ALLOW_LDS((contract_inv_mxs<4, 3>));
-    ALLOW_LDS((contract_inv_mxs<2, 2>));
-    ALLOW_LDS((contract_inv_mxs<2, 3>));
 #undef ALLOW_LDS
     kernel_attrs_done = true;
     return FFTSO3_OK;

@@ -1,18 +1,21 @@
-// mfma_split.hpp -- batch-wide beta contraction in split-BF16 on MFMA (v_mfma_f32_32x32x16_bf16),
-// N = 2B <= 128.
+// mfma_split.hpp -- batch-wide beta contractions in split-BF16 on MFMA (v_mfma_f32_32x32x16_bf16):
+// every product formed from bf16 pieces of both operands and accumulated in FP32 (BF16X3: 2 pieces,
+// 3 products; BF16X6: 3 pieces, 6 products; see mfma.hpp).  The Wigner-d values come from the FP64
+// recursion, split once per work item into pieces in LDS.
 //
-// Same work decomposition, workspace layout and coefficient staging (PT) as the FP32 kernels
-// contract_fwd_mx32 / contract_inv_mx32 (one workgroup per (m,n) group for the whole batch, the
-// batch in blocks of 128 spectrum columns), with every product formed from bf16 pieces of both
-// operands and accumulated in FP32 (BF16X3: 2 pieces, 3 products; BF16X6: 3 pieces, 6 products;
-// see mfma.hpp).  The Wigner-d block is generated by the FP64 recursion and split once into
-// pieces in LDS; each spectrum (forward) or coefficient (inverse) block is split in registers.
+// contract_fwd_mxs_body / contract_inv_mxs_body (N <= 128) are the units of the unified schedule
+// of uneven plans (multi.hpp): the work decomposition, workspace layout and coefficient staging
+// (PT) of the FP32 kernels contract_fwd_mx32 / contract_inv_mx32 (one workgroup per (m,n) group
+// for the whole batch, the batch in blocks of 128 spectrum columns).  The reduction index of one
+// instruction is 16 values: lane i supplies 8 of them for row (A) or column (B) i % 32; the beta
+// (forward) or degree (inverse) index r of a lane's j-th value in instruction t is
+// r = 2(8t + j) + i/32, the pairing the FP32 kernels use for their 2-deep instructions, so both
+// operands read the same LDS rows as there.  The pieces of D are stored in instruction order,
+// [piece][row][t][i/32][j], so a lane reads its 8 values as 16 bytes.
 //
-// The reduction index of one instruction is 16 values: lane i supplies 8 of them for row (A) or
-// column (B) i % 32.  The beta (forward) or degree (inverse) index r of a lane's j-th value in
-// instruction t is r = 2(8t + j) + i/32, the pairing the FP32 kernels use for their 2-deep
-// instructions, so both operands read the same LDS rows as there.  The pieces of D are stored
-// in instruction order, [piece][row][t][i/32][j], so a lane reads its 8 values as 16 bytes.
+// contract_fwd_mxks / contract_inv_mxks (N <= 256, below) are the launches of every batch-wide
+// split plan of its own: each lane loads its spectrum (forward) or coefficient (inverse) operands
+// straight into registers, one step ahead, and the block loop has no barrier.
 #pragma once
 
 #include "mfma.hpp"
@@ -166,11 +169,6 @@ __device__ __forceinline__ void contract_fwd_mxs_body(const MxArgs &a, int unit)
     }
 }
 
-template <int NP, int NPC>
-__global__ void __launch_bounds__(256) contract_fwd_mxs(MxArgs a)
-{
-    contract_fwd_mxs_body<NP, NPC>(a, mx_block_unit(a));
-}
 
 // Inverse: G_k(fn, pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) PT(packed(l, pair); fn), split-BF16
 // (c_l = 2l+1 for SO(3), 1 for S^2).
@@ -297,16 +295,11 @@ __device__ __forceinline__ void contract_inv_mxs_body(const MxArgs &a, int unit)
     }
 }
 
-template <int NP, int NPC>
-__global__ void __launch_bounds__(256) contract_inv_mxs(MxArgs a)
-{
-    contract_inv_mxs_body<NP, NPC>(a, mx_block_unit(a));
-}
 
 // ---------------------------------------------------------------------------------------------
-// Batch-wide split-BF16 contractions for 128 < N <= 256 (S^2 up to l_max = 127, SO(3) from
-// l_max = 64 to 127): the FP32 K-chunked kernels' scheme (contract_fwd_mxk / contract_inv_mxk) with
-// bf16 pieces.  A group's whole d block in pieces (6 bytes per value in bf16x6) does not fit in LDS
+// Batch-wide split-BF16 contractions for N <= 256 (S^2 up to l_max = 127, SO(3) up to l_max = 127;
+// the unified schedule of uneven plans keeps contract_fwd_mxs / contract_inv_mxs above): the FP32
+// K-chunked kernels' scheme (contract_fwd_mxk / contract_inv_mxk) with bf16 pieces.  A group's whole d block in pieces (6 bytes per value in bf16x6) does not fit in LDS
 // at B = 128, so each work item covers part of it: the forward a range of up to 64 degrees (the
 // item's spectrum blocks are read once per range), the inverse a range of up to 128 beta values
 // (its coefficient blocks are read once per range).  Within an item the reduction index of one
@@ -325,6 +318,15 @@ __host__ __device__ inline size_t mxks_inv_lds_bytes(int npc, int NP)
     return (size_t)npc * MXKS_KR * (MXK_MAXROWS + 8) * 2 + (size_t)NP * MXK_MAXROWS * 4;
 }
 // forward items 2 g + h (degrees m0 + 64 h ..), inverse items g * ceil(N / 128) + kr (beta 128 kr ..)
+
+// offset within a workspace row of block column c = (function, pair, re/im) of group g: with group
+// interleave (ws_rec) the records of 2^gil consecutive groups sit side by side per function
+template <int NP>
+__device__ __forceinline__ long mxks_col(const ChunkArgs &ca, int g, int c)
+{
+    const int fn = c / (2 * NP), rem = c - fn * 2 * NP;
+    return (long)((fn << ca.gil) + (g & ((1 << ca.gil) - 1))) * (2 * NP) + rem;
+}
 
 // seed d^{m0}_{m0 n0}(beta_k) (S^2: nP^{m0}_{m0}) of group g, as group_recursion_lds
 __device__ __forceinline__ double mxks_seed(const RecTables &t, int s2, int g, int m0, int n0, int k)
@@ -397,12 +399,13 @@ __device__ __forceinline__ void contract_fwd_mxks_body(const MxArgs &a, int unit
     const float *S = reinterpret_cast<const float *>(a.src);
     float *PT = reinterpret_cast<float *>(a.dst);
     const unsigned short *Da = Dp + cl * DPK + 8 * kh;
+    const int gr = g & ~((1 << ca.gil) - 1);   // the workspace row group of g's interleaved records
     for (int fb = fb0; fb < fb1; fb++) {
         const int c = fb * 32 * NW + cb, cv = c < NCOL ? c : 0;
-        const float *src = S + cv;
+        const float *src = S + mxks_col<NP>(ca, g, cv);
         auto load = [&](float *x, int st) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) x[j] = src[ws_row(ca, g, min(16 * st + 8 * kh + j, N - 1)) * CW];
+            for (int j = 0; j < 8; j++) x[j] = src[ws_row(ca, gr, min(16 * st + 8 * kh + j, N - 1)) * CW];
         };
         f32x16_t acc0, acc1;
 #pragma unroll
@@ -516,8 +519,10 @@ __device__ __forceinline__ void contract_inv_mxks_body(const MxArgs &a, int unit
     const float *PT = reinterpret_cast<const float *>(a.src);
     float *G = reinterpret_cast<float *>(a.dst);
     const int *et = etab + p * MXK_MAXROWS + 8 * kh;
+    const int gr = g & ~((1 << ca.gil) - 1);   // the workspace row group of g's interleaved records
     for (int fb = fb0; fb < fb1; fb++) {
         const int c = fb * 32 * NW + cb, cv = c < NCOL ? c : 0;
+        const long gcol = mxks_col<NP>(ca, g, cv);
         const float *src = PT + 2 * (cv / (2 * NP)) + (cv & 1);
         f32x16_t acc[4];
 #pragma unroll
@@ -557,7 +562,7 @@ __device__ __forceinline__ void contract_inv_mxks_body(const MxArgs &a, int unit
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
                     const int kk = 32 * t + 8 * (j >> 2) + 4 * kh + (j & 3);
-                    float *d = ok && kk < kn ? G + ws_row(ca, g, k0 + kk) * CW + c : a.dummy;
+                    float *d = ok && kk < kn ? G + ws_row(ca, gr, k0 + kk) * CW + gcol : a.dummy;
                     *d = sg * acc[t][j];
                 }
             }

@@ -50,9 +50,11 @@ def test_r16_slice_fft_uniform(fftso3, torch_cuda, oracle, B, kind):
                                             ("complex", "bf16x3")])
 def test_unified_matches_per_class_and_oracle(fftso3, torch_cuda, oracle, kind, precision):
     """Every bandlimit of config 4 ({8,16,24,32,48,64}) in one plan, 3 to 5 functions each in
-    mixed order: unified launches (one per stage) equal the per-class launches bit for bit, and one
-    function of every bandlimit matches the oracle.  The split-BF16 precisions run their one-group
-    contraction kernels in the unified launches (bf16x3 bar 5e-5)."""
+    mixed order: unified launches (one per stage) equal the per-class launches (FP32: bit for bit
+    below l_max = 63, where both run the same kernel bodies; the split precisions run the
+    one-group bodies in the unified launches and the K-chunked split kernels per class, so those
+    agree to the precision's bar), and one function of every bandlimit matches the oracle
+    (bf16x3 bar 5e-5)."""
     torch = torch_cuda
     cplx = kind == "complex"
     Bs = [64, 8, 48, 16, 16, 32, 24, 64, 8, 48, 24, 32, 8, 16, 64, 48, 24, 32, 8, 64, 16, 24]
@@ -82,10 +84,11 @@ def test_unified_matches_per_class_and_oracle(fftso3, torch_cuda, oracle, kind, 
     for i, B in enumerate(Bs):
         gs = slice(go[i], go[i] + (2 * B) ** 3)
         cs = slice(co[i], co[i] + len(Fs[i]))
-        if B < 64:
+        if B < 64 and precision == "fp32":
             assert np.array_equal(g0[gs], g1[gs]) and np.array_equal(c0[cs], c1[cs]), (i, B)
         else:
-            assert rel_l2(g0[gs], g1[gs]) < 1e-6 and rel_l2(c0[cs], c1[cs]) < 1e-6, (i, B)
+            bar = 2e-5 if precision == "bf16x3" else 1e-6
+            assert rel_l2(g0[gs], g1[gs]) < bar and rel_l2(c0[cs], c1[cs]) < bar, (i, B)
     seen = set()
     for i, B in enumerate(Bs):
         ci = c0[co[i]:co[i] + len(Fs[i])]
