@@ -971,13 +971,17 @@ __device__ __forceinline__ void contract_fwd_mxk_body(const MxArgs &a, int unit)
         load(x, 0);
         for (int s0 = 0; s0 < NS; s0 += 8) {
             if (s0 + 8 < NS) load(xn, s0 + 8);
+            // the group's 8 x nrt operands of D read from LDS ahead of their MFMAs (NS is a multiple
+            // of 16, so every group is whole)
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                if (s0 + u < NS) {
+            for (int t = 0; t < 4; t++) {
+                if (t < nrt) {
+                    float d[8];
 #pragma unroll
-                    for (int t = 0; t < 4; t++)
-                        if (t < nrt)
-                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(Dw[32 * t * DP + 2 * (s0 + u)], x[u], acc[t], 0, 0, 0);
+                    for (int u = 0; u < 8; u++) d[u] = Dw[32 * t * DP + 2 * (s0 + u)];
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < 8; u++) acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], x[u], acc[t], 0, 0, 0);
                 }
             }
 #pragma unroll
