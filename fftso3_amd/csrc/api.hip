@@ -379,7 +379,7 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
         const int npc = pl->prec == FFTSO3_BF16X6 ? 3 : 2;
         c->mxks = split && N > 128 && B <= MXK_MAXROWS && !pl->force_chunked &&
                   mxks_fwd_lds_bytes(N, npc, np) + rcb <= (size_t)kMaxLds &&
-                  mxks_inv_lds_bytes(npc, np) + rcb <= (size_t)kMaxLds;
+                  mxks_inv_lds_bytes(npc, np, B) + rcb <= (size_t)kMaxLds;
         c->mx = ((pl->prec == FFTSO3_FP32 || split) && N <= 128 && !pl->force_chunked) || c->mxk || c->mxks;
     }
     c->mx64 = mx64_eligible(pl, c);
@@ -913,9 +913,9 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             ai.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + ai.nitems - 1) / ai.nitems)));
             const int igrid = deal_grid(ai.nitems * ai.nsplit, ai.nsplit << ca.gil);
             if (pl->prec == FFTSO3_BF16X6)
-                hipLaunchKernelGGL((contract_inv_mxks<NP, 3>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(3, NP) + rcb, s, ai);
+                hipLaunchKernelGGL((contract_inv_mxks<NP, 3>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(3, NP, c->B) + rcb, s, ai);
             else
-                hipLaunchKernelGGL((contract_inv_mxks<NP, 2>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(2, NP) + rcb, s, ai);
+                hipLaunchKernelGGL((contract_inv_mxks<NP, 2>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(2, NP, c->B) + rcb, s, ai);
         } else if (c->mxk)
         {
             // 8 waves per workgroup over blocks of 256 columns: half as many units

@@ -313,9 +313,11 @@ __host__ __device__ inline size_t mxks_fwd_lds_bytes(int N, int npc, int NP)
 {
     return (size_t)npc * MXKS_LR * mxks_dpitch(N) * 2 + (size_t)NP * MXK_MAXROWS * 4;
 }
-__host__ __device__ inline size_t mxks_inv_lds_bytes(int npc, int NP)
+// inverse: D^T pieces [npc][128][M16 + 8] for the largest group (M16 = B rounded up to 16)
+__host__ __device__ inline int mxks_inv_pitch(int B) { return ((B + 15) & ~15) + 8; }
+__host__ __device__ inline size_t mxks_inv_lds_bytes(int npc, int NP, int B)
 {
-    return (size_t)npc * MXKS_KR * (MXK_MAXROWS + 8) * 2 + (size_t)NP * MXK_MAXROWS * 4;
+    return (size_t)npc * MXKS_KR * mxks_inv_pitch(B) * 2 + (size_t)NP * MXK_MAXROWS * 4;
 }
 // forward items 2 g + h (degrees m0 + 64 h ..), inverse items g * ceil(N / 128) + kr (beta 128 kr ..)
 
@@ -472,8 +474,8 @@ __device__ __forceinline__ void contract_inv_mxks_body(const MxArgs &a, int unit
     const int M = B - m0, M16 = (M + 15) & ~15, LP = M16 + 8, nu = M16 / 16;
     const int kn = min(MXKS_KR, N - k0), nkt = (kn + 31) / 32;
     unsigned short *Dt = reinterpret_cast<unsigned short *>(smem_raw);                          // [NPC][128][LP]
-    int *etab = reinterpret_cast<int *>(smem_raw + (size_t)NPC * MXKS_KR * (MXK_MAXROWS + 8) * 2); // [NP][MXK_MAXROWS]
-    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxks_inv_lds_bytes(NPC, NP));
+    int *etab = reinterpret_cast<int *>(smem_raw + (size_t)NPC * MXKS_KR * mxks_inv_pitch(B) * 2); // [NP][MXK_MAXROWS]
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxks_inv_lds_bytes(NPC, NP, B));
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     for (int i = tid; i < NP * MXK_MAXROWS; i += 64 * NW) {
@@ -561,7 +563,10 @@ __device__ __forceinline__ void contract_inv_mxks_body(const MxArgs &a, int unit
             if (t < nkt) {
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
-                    const int kk = 32 * t + 8 * (j >> 2) + 4 * kh + (j & 3);
+                    int kk = 32 * t + 8 * (j >> 2) + 4 * kh + (j & 3);
+                    // opaque here, so the 64 row addresses are formed at the stores instead of being
+                    // hoisted above the degree loop and held in registers across it
+                    asm volatile("" : "+v"(kk));
                     float *d = ok && kk < kn ? G + ws_row(ca, gr, k0 + kk) * CW + gcol : a.dummy;
                     *d = sg * acc[t][j];
                 }
