@@ -17,6 +17,7 @@ DOMAIN = {"so3": 0, "s2": 1}
 KIND = {"complex": 0, "real": 1}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
 FLAG_PER_CLASS_LAUNCHES = 1
+FLAG_PER_CLASS_CONTRACTIONS = 2
 STAGES = ("fft_fwd", "contract_fwd", "basis_fwd", "basis_inv", "contract_inv", "fft_inv")
 
 _lib = None

@@ -178,6 +178,7 @@ struct fftso3_plan_s {
     // unified schedule: every bandlimit class in one launch per stage (multi.hpp), each class in
     // its own region of the workspaces; the queues of (entry, unit) items per stage
     bool multi_on = true;              // allowed (autotune may switch it off)
+    bool uni_pc = false;               // unified FFT and layout stages, contractions launched per class
     bool unified = false;              // in use for the current schedule
     DevBuf d_q_fft, d_q_cf, d_q_ci, d_q_tr;
     int nq_fft = 0, nq_cf = 0, nq_ci = 0, nq_tr = 0;
@@ -794,11 +795,13 @@ static int fused_threads(int N)
     return std::max(64, std::min(kFftThreads, t));
 }
 
+// The batch-wide contraction of one class (forward: spectrum workspace src -> PT dst; inverse: PT
+// src -> workspace dst), with the kernels and work split its own launches use; run_mx and the
+// unified schedule's per-class contraction stages call it.
 template <int NP>
-int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *in, void *out, hipStream_t s)
+int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *src, void *dst,
+                       hipStream_t s)
 {
-    const bool real = pl->kind == FFTSO3_REAL;
-    const bool s2 = NP == 2;
     MxArgs a;
     a.ca = ca;
     a.rt = rec_tables(c);
@@ -815,96 +818,59 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     const size_t rcb = (size_t)c->B * sizeof(RecCoef);   // per-group recursion coefficients
     const size_t lds = MX_LDS_BYTES + rcb;
     a.dummy = reinterpret_cast<float *>(pl->d_dummy.p);
-    const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
-    const int tgrid = coef_transpose_grid<float2>(ncp, ca.nfc);
-    // the real path stages packed complex coefficients in d_tmp, addressed per class-local function
-    ChunkArgs ct = ca;
-    if (real) {
-        ct.fn = reinterpret_cast<const int *>(c->d_ident.p);
-        ct.coef_off = reinterpret_cast<const long *>(c->d_ws_off.p);
-    }
-    const long nc = ncp;
-    const int tmix_blocks = (int)std::min<long>(4096, (nc * ca.nfc + 255) / 256);
+    a.src = src;
+    a.dst = dst;
     if (fwd) {
-        {
-            StageTimer t(pl, s, 1);
-            a.src = pl->d_ws.p;
-            a.dst = pl->d_fc.p;
-            if (pl->prec == FFTSO3_BF16X6 || pl->prec == FFTSO3_BF16X3) {
-                MxArgs af = a;
-                af.items = (const int *)c->d_items_mxs_f.p;
-                af.nitems = c->nitems_mxs_f;
-                const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
-                af.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + af.nitems - 1) / af.nitems)));
-                const int fgrid = deal_grid(af.nitems * af.nsplit, af.nsplit << ca.gil);
-                if (pl->prec == FFTSO3_BF16X6)
-                    hipLaunchKernelGGL((contract_fwd_mxks<NP, 3>), dim3(fgrid), dim3(512),
-                                       mxks_fwd_lds_bytes(c->N, 3, NP) + rcb, s, af);
-                else
-                    hipLaunchKernelGGL((contract_fwd_mxks<NP, 2>), dim3(fgrid), dim3(512),
-                                       mxks_fwd_lds_bytes(c->N, 2, NP) + rcb, s, af);
-            } else if (c->mxk) {
-                // 8 waves per workgroup over blocks of 256 columns: half as many units
-                MxArgs a8 = a;
-                const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
-                a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
-                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
-                                   dim3(512), mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a8);
+        if (pl->prec == FFTSO3_BF16X6 || pl->prec == FFTSO3_BF16X3) {
+            MxArgs af = a;
+            af.items = (const int *)c->d_items_mxs_f.p;
+            af.nitems = c->nitems_mxs_f;
+            const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+            af.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + af.nitems - 1) / af.nitems)));
+            const int fgrid = deal_grid(af.nitems * af.nsplit, af.nsplit << ca.gil);
+            if (pl->prec == FFTSO3_BF16X6)
+                hipLaunchKernelGGL((contract_fwd_mxks<NP, 3>), dim3(fgrid), dim3(512),
+                                   mxks_fwd_lds_bytes(c->N, 3, NP) + rcb, s, af);
+            else
+                hipLaunchKernelGGL((contract_fwd_mxks<NP, 2>), dim3(fgrid), dim3(512),
+                                   mxks_fwd_lds_bytes(c->N, 2, NP) + rcb, s, af);
+        } else if (c->mxk) {
+            // 8 waves per workgroup over blocks of 256 columns: half as many units
+            MxArgs a8 = a;
+            const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
+            a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
+            hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
+                               dim3(512), mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a8);
+        }
+        else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2) {
+            // the 2^gil groups that share workspace lines in one workgroup
+            MxArgs ag = a;
+            ag.gpw = 1 << ca.gil;
+            ag.nitems = c->nitems_mx / ag.gpw;
+            const int fbw = 16 / ag.gpw, nfbg = (ca.nfc + fbw - 1) / fbw;
+            ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
+            const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
+            const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
+            if (ag.gpw == 2) {
+                // groups with at most 32 degrees (m0 >= B - 32; 3/4 of them at l_max = 63) keep
+                // two 16-row tiles of D in registers instead of four: fewer registers, two
+                // workgroups per CU
+                const int gsplit = c->B > 32 ? (c->B - 32) * (c->B - 32) : 0;
+                MxArgs a4 = ag, a2 = ag;
+                a4.nitems = gsplit / 2;
+                a2.g0 = gsplit;
+                a2.nitems = ag.nitems - gsplit / 2;
+                if (a4.nitems > 0)
+                    hipLaunchKernelGGL((contract_fwd_mx16g<4>), dim3(deal_grid(a4.nitems * a4.nsplit, a4.nsplit)),
+                                       dim3(256), glds, s, a4);
+                if (a2.nitems > 0)
+                    hipLaunchKernelGGL((contract_fwd_mx16g<2>), dim3(deal_grid(a2.nitems * a2.nsplit, a2.nsplit)),
+                                       dim3(256), glds, s, a2);
             }
-            else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2) {
-                // the 2^gil groups that share workspace lines in one workgroup
-                MxArgs ag = a;
-                ag.gpw = 1 << ca.gil;
-                ag.nitems = c->nitems_mx / ag.gpw;
-                const int fbw = 16 / ag.gpw, nfbg = (ca.nfc + fbw - 1) / fbw;
-                ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
-                const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
-                const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
-                if (ag.gpw == 2) {
-                    // groups with at most 32 degrees (m0 >= B - 32; 3/4 of them at l_max = 63) keep
-                    // two 16-row tiles of D in registers instead of four: fewer registers, two
-                    // workgroups per CU
-                    const int gsplit = c->B > 32 ? (c->B - 32) * (c->B - 32) : 0;
-                    MxArgs a4 = ag, a2 = ag;
-                    a4.nitems = gsplit / 2;
-                    a2.g0 = gsplit;
-                    a2.nitems = ag.nitems - gsplit / 2;
-                    if (a4.nitems > 0)
-                        hipLaunchKernelGGL((contract_fwd_mx16g<4>), dim3(deal_grid(a4.nitems * a4.nsplit, a4.nsplit)),
-                                           dim3(256), glds, s, a4);
-                    if (a2.nitems > 0)
-                        hipLaunchKernelGGL((contract_fwd_mx16g<2>), dim3(deal_grid(a2.nitems * a2.nsplit, a2.nsplit)),
-                                           dim3(256), glds, s, a2);
-                }
-                else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
-            } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
-            else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
-            HIP_TRY(hipGetLastError());
-        }
-        StageTimer t(pl, s, 2);
-        hipLaunchKernelGGL((coef_transpose<0, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, pl->d_fc.p,
-                           real ? pl->d_tmp.p : out);
-        HIP_TRY(hipGetLastError());
-        if (real) {
-            if (s2) hipLaunchKernelGGL((s2_tmix<float, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_tmp.p, out, nc);
-            else hipLaunchKernelGGL((so3_tmix<float, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_tmp.p, out, nc);
-            HIP_TRY(hipGetLastError());
-        }
+            else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
+        } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
+        else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
     } else {
-        {
-            StageTimer t(pl, s, 3);
-            if (real) {
-                if (s2) hipLaunchKernelGGL((s2_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
-                else hipLaunchKernelGGL((so3_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
-                HIP_TRY(hipGetLastError());
-            }
-            hipLaunchKernelGGL((coef_transpose<1, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
-                               pl->d_fc.p);
-            HIP_TRY(hipGetLastError());
-        }
-        StageTimer t(pl, s, 4);
-        a.src = pl->d_fc.p;
-        a.dst = pl->d_ws.p;
         if (pl->prec == FFTSO3_BF16X6 || pl->prec == FFTSO3_BF16X3) {
             MxArgs ai = a;
             ai.items = (const int *)c->d_items_mxs_i.p;
@@ -936,7 +902,55 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
             hipLaunchKernelGGL((contract_inv_mx32g<2>), dim3(deal_grid(ag.nitems * ag.nsplit, ag.nsplit)), dim3(256),
                                glds, s, ag);
         } else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return FFTSO3_OK;
+}
+
+template <int NP>
+int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const void *in, void *out, hipStream_t s)
+{
+    const bool real = pl->kind == FFTSO3_REAL;
+    const bool s2 = NP == 2;
+    const int ncp = s2 ? c->B * c->B : (int)so3_ncoef(c->B);
+    const int tgrid = coef_transpose_grid<float2>(ncp, ca.nfc);
+    // the real path stages packed complex coefficients in d_tmp, addressed per class-local function
+    ChunkArgs ct = ca;
+    if (real) {
+        ct.fn = reinterpret_cast<const int *>(c->d_ident.p);
+        ct.coef_off = reinterpret_cast<const long *>(c->d_ws_off.p);
+    }
+    const long nc = ncp;
+    const int tmix_blocks = (int)std::min<long>(4096, (nc * ca.nfc + 255) / 256);
+    int st;
+    if (fwd) {
+        {
+            StageTimer t(pl, s, 1);
+            if ((st = launch_mx_contract<NP>(pl, c, ca, true, pl->d_ws.p, pl->d_fc.p, s))) return st;
+        }
+        StageTimer t(pl, s, 2);
+        hipLaunchKernelGGL((coef_transpose<0, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, pl->d_fc.p,
+                           real ? pl->d_tmp.p : out);
         HIP_TRY(hipGetLastError());
+        if (real) {
+            if (s2) hipLaunchKernelGGL((s2_tmix<float, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_tmp.p, out, nc);
+            else hipLaunchKernelGGL((so3_tmix<float, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_tmp.p, out, nc);
+            HIP_TRY(hipGetLastError());
+        }
+    } else {
+        {
+            StageTimer t(pl, s, 3);
+            if (real) {
+                if (s2) hipLaunchKernelGGL((s2_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
+                else hipLaunchKernelGGL((so3_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
+                HIP_TRY(hipGetLastError());
+            }
+            hipLaunchKernelGGL((coef_transpose<1, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
+                               pl->d_fc.p);
+            HIP_TRY(hipGetLastError());
+        }
+        StageTimer t(pl, s, 4);
+        if ((st = launch_mx_contract<NP>(pl, c, ca, false, pl->d_fc.p, pl->d_ws.p, s))) return st;
     }
     return FFTSO3_OK;
 }
@@ -1319,6 +1333,15 @@ MxArgs make_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, int k
     const int nfb = (ca.nfc * 4 * 2 + 127) / 128;
     a.nsplit = std::max(1, std::min(nfb, pl->split_mul * ((2048 + a.nitems - 1) / a.nitems)));
     if (kind == 0) return a;
+    if (kind == 4) {
+        // split kernels (as run_mx launches them): items of (group, degree / beta range), blocks of
+        // 256 columns
+        a.items = (const int *)(fwd ? c->d_items_mxs_f.p : c->d_items_mxs_i.p);
+        a.nitems = fwd ? c->nitems_mxs_f : c->nitems_mxs_i;
+        const int nfb8 = (ca.nfc * 4 * 2 + 255) / 256;
+        a.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
+        return a;
+    }
     a.gpw = 2;
     a.nitems = c->nitems_mx / 2;
     const int nfbg = (ca.nfc + 7) / 8;
@@ -1403,18 +1426,20 @@ int build_unified(fftso3_plan_s *pl)
         const ChunkArgs ca = chunk_args<float>(pl, c, 0, nfc);
         qf.add_units(i, fft_units(c));
         qtr.add_units(i, coef_transpose_units<float2>(so3_ncoef(c->B), nfc));
-        // split-BF16 plans run the one-group kernels (kind 0) at every bandlimit
+        // split-BF16 plans run the split kernels (kind 4) at every bandlimit
+        const bool split = pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6;
         const bool grouped = c->N == 128 && pl->prec == FFTSO3_FP32;
-        const std::vector<int> kf = grouped ? std::vector<int>{2, 3} : std::vector<int>{0};
+        const std::vector<int> kf = split ? std::vector<int>{4} : grouped ? std::vector<int>{2, 3} : std::vector<int>{0};
         for (int kind : kf) {
             const MxArgs a = make_mx(pl, c, ca, true, kind);
             if (a.nitems <= 0) continue;
             qcf.add_dealt((int)pl->ent_cf.size(), a.nitems * a.nsplit, a.gpw > 1 ? a.nsplit : a.nsplit << ca.gil);
             pl->ent_cf.push_back({c, kind});
         }
-        const MxArgs ai = make_mx(pl, c, ca, false, grouped ? 1 : 0);
+        const int ki = split ? 4 : grouped ? 1 : 0;
+        const MxArgs ai = make_mx(pl, c, ca, false, ki);
         qci.add_dealt((int)pl->ent_ci.size(), ai.nitems * ai.nsplit, ai.gpw > 1 ? ai.nsplit : ai.nsplit << ca.gil);
-        pl->ent_ci.push_back({c, grouped ? 1 : 0});
+        pl->ent_ci.push_back({c, ki});
     }
     if (pl->ent_cf.size() > MULTI_MAX_MX || pl->ent_ci.size() > MULTI_MAX_MX)
         return fail(FFTSO3_EUNSUPPORTED, "too many contraction entries for one launch");
@@ -1478,6 +1503,19 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
     // split-BF16 pieces per operand (0: FP32 MFMA kernels)
     const int npc = pl->prec == FFTSO3_BF16X6 ? 3 : pl->prec == FFTSO3_BF16X3 ? 2 : 0;
     auto contract_stage = [&](bool f) -> int {
+        if (pl->uni_pc) {
+            // each class's own contraction launch on its region of the workspaces: a persistent
+            // kernel's next unit waits on the previous unit's stores (loads and stores share one
+            // counter), which separate launches do not
+            for (int i = 0; i < ncls; i++) {
+                Class *c = pl->uorder[i];
+                const void *src = f ? (const void *)(ws + c->ws_off) : (const void *)(fc + c->fc_off);
+                void *dst = f ? (void *)(fc + c->fc_off) : (void *)(ws + c->ws_off);
+                int st = launch_mx_contract<4>(pl, c, ca[i], f, src, dst, s);
+                if (st) return st;
+            }
+            return FFTSO3_OK;
+        }
         MultiMx m;
         std::memset(&m, 0, sizeof(m));
         const auto &ent = f ? pl->ent_cf : pl->ent_ci;
@@ -1491,7 +1529,7 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
             m.c[e] = a;
             m.kind[e] = ent[e].second;
             const size_t rcb = (size_t)c->B * sizeof(RecCoef);
-            const size_t l = npc ? (f ? mxs_fwd_lds_bytes(npc) : mxs_inv_lds_bytes(npc)) + rcb
+            const size_t l = npc ? (f ? mxks_fwd_lds_bytes(c->N, npc, 4) : mxks_inv_lds_bytes(npc, 4, c->B)) + rcb
                            : ent[e].second == 0 ? MX_LDS_BYTES + rcb
                            : f ? (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb
                                : (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
@@ -1501,15 +1539,15 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
         m.q.ctr = reinterpret_cast<unsigned *>(pl->d_ctr.p) + (f ? 16 : 32);
         std::memcpy(m.q.len, f ? pl->ql_cf : pl->ql_ci, sizeof(m.q.len));
         const int nitems = f ? pl->nq_cf : pl->nq_ci;
-#define MULTI_LAUNCH(K) hipLaunchKernelGGL(K, dim3(persistent_grid(K, lds, nitems, 256)), dim3(256), lds, s, m)
+#define MULTI_LAUNCH(K, T) hipLaunchKernelGGL(K, dim3(persistent_grid(K, lds, nitems, T)), dim3(T), lds, s, m)
         if (npc == 3) {
-            if (f) MULTI_LAUNCH(contract_fwd_multi_s<3>);
-            else MULTI_LAUNCH(contract_inv_multi_s<3>);
+            if (f) MULTI_LAUNCH(contract_fwd_multi_s<3>, 512);
+            else MULTI_LAUNCH(contract_inv_multi_s<3>, 512);
         } else if (npc == 2) {
-            if (f) MULTI_LAUNCH(contract_fwd_multi_s<2>);
-            else MULTI_LAUNCH(contract_inv_multi_s<2>);
-        } else if (f) MULTI_LAUNCH(contract_fwd_multi);
-        else MULTI_LAUNCH(contract_inv_multi);
+            if (f) MULTI_LAUNCH(contract_fwd_multi_s<2>, 512);
+            else MULTI_LAUNCH(contract_inv_multi_s<2>, 512);
+        } else if (f) MULTI_LAUNCH(contract_fwd_multi, 256);
+        else MULTI_LAUNCH(contract_inv_multi, 256);
 #undef MULTI_LAUNCH
         HIP_TRY(hipGetLastError());
         return FFTSO3_OK;
@@ -1741,7 +1779,8 @@ int set_kernel_attrs()
 
 int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int kind, int precision, unsigned flags)
 {
-    if (flags & ~(unsigned)FFTSO3_FLAG_PER_CLASS_LAUNCHES) return fail(FFTSO3_EINVAL, "unknown plan flags 0x%x", flags);
+    if (flags & ~(unsigned)(FFTSO3_FLAG_PER_CLASS_LAUNCHES | FFTSO3_FLAG_PER_CLASS_CONTRACTIONS))
+        return fail(FFTSO3_EINVAL, "unknown plan flags 0x%x", flags);
     if (!out) return fail(FFTSO3_EINVAL, "plan pointer is NULL");
     *out = nullptr;
     if (domain != FFTSO3_DOMAIN_SO3 && domain != FFTSO3_DOMAIN_S2) return fail(FFTSO3_EINVAL, "unknown domain %d", domain);
@@ -1761,6 +1800,7 @@ int create_common(fftso3_plan_t *out, int domain, const int *lmax, int n_fn, int
     if (!pl) return fail(FFTSO3_ENOMEM, "host allocation failed");
     pl->device = dev;
     pl->multi_on = !(flags & FFTSO3_FLAG_PER_CLASS_LAUNCHES);
+    pl->uni_pc = (flags & FFTSO3_FLAG_PER_CLASS_CONTRACTIONS) != 0;
     pl->domain = domain;
     pl->kind = kind;
     pl->prec = precision;
@@ -1862,7 +1902,7 @@ int fftso3_plan_describe(fftso3_plan_t plan, char *buf, int len)
         ChunkArgs ca = chunk_args<float>(plan, const_cast<Class *>(c), 0, (int)c->fns.size());
         std::snprintf(line, sizeof(line), "l_max=%d fns=%d %s %s chunk=%d klo=%d gil=%d split=%d%s%s%s%s%s%s\n", c->B - 1,
                       (int)c->fns.size(), prec, path, c->chunk, c->mx ? plan->klo : 0, ca.gil, plan->split_mul,
-                      c->r16 ? " fft16R" : "", plan->unified ? " unified" : "", c->d_dtab.p ? " d-table" : "",
+                      c->r16 ? " fft16R" : "", plan->unified ? (plan->uni_pc ? " unified (contractions per class)" : " unified") : "", c->d_dtab.p ? " d-table" : "",
                       c->mx64 ? " mfma64" : "", c->otf ? " d-on-the-fly" : "", plan->graph ? " graph" : "");
         out += line;
     }
@@ -2046,7 +2086,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     if (!d_grid || !d_coef) return fail(FFTSO3_EINVAL, "NULL device pointer");
     if (reps < 1 || reps > 100) return fail(FFTSO3_EINVAL, "reps must be in [1, 100] (got %d)", reps);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    struct Cand { bool chunked; int C, klo, split, gil; bool multi = true; };
+    struct Cand { bool chunked; int C, klo, split, gil; bool multi = true; bool pc = false; };
     std::vector<Cand> cands;
     const bool bf16 = plan->prec == FFTSO3_BF16X3 || plan->prec == FFTSO3_BF16X6;
     if (!bf16) {
@@ -2055,9 +2095,13 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
                 for (int sp : {1, 2, 4})
                     for (int gil : {0, 1, 2})
                         if (klo == 0 || gil == 0) cands.push_back({false, plan->C, klo, sp, gil});
-        // several bandlimit classes: also one launch per class and stage instead of the unified launches
+        // several bandlimit classes: also one launch per class and stage instead of the unified
+        // launches, and unified FFT stages with per-class contraction launches
         if (plan->prec == FFTSO3_FP32 && plan->classes.size() > 1)
-            for (int sp : {1, 2}) cands.push_back({false, plan->C, 0, sp, 1, false});
+            for (int sp : {1, 2}) {
+                cands.push_back({false, plan->C, 0, sp, 1, false});
+                cands.push_back({false, plan->C, 0, sp, 1, true, true});
+            }
         // FP64: the batch-wide MFMA contractions with groups split over 1, 2 or 4 workgroups
         if (plan->prec == FFTSO3_FP64)
             for (int sp : {1, 2, 4}) cands.push_back({false, plan->C, 0, sp, 0});
@@ -2067,6 +2111,8 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         // the chunked kernels (8 functions per pass)
         for (int sp : {1, 2, 4})
             for (int gil : {0, 1}) cands.push_back({false, 8, 0, sp, gil});
+        if (plan->classes.size() > 1)
+            for (int sp : {1, 2}) cands.push_back({false, 8, 0, sp, 1, true, true});
         cands.push_back({true, 8, 0, 1, 0});
     }
     if (cands.empty()) return FFTSO3_OK;
@@ -2086,6 +2132,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         plan->split_mul = c.split;
         plan->gil = c.gil;
         plan->multi_on = c.multi;
+        plan->uni_pc = c.pc;
         if ((st = finalize_plan(plan))) break;
         if ((st = run_dispatch(plan, false, d_coef, d_grid, s)) || (st = run_dispatch(plan, true, d_grid, d_coef, s))) break;
         (void)hipEventRecord(e0, s);
@@ -2108,6 +2155,7 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
     plan->split_mul = win.split;
     plan->gil = win.gil;
     plan->multi_on = win.multi;
+    plan->uni_pc = win.pc;
     const int st2 = finalize_plan(plan);
     plan->timing = timing;
     plan->graph = graph;

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(1024) so3_fft_multi(MultiFft m)
 
 // contraction kinds: forward 0 = contract_fwd_mx32<4, false> (N < 128), 2 = contract_fwd_mx16g<4>,
 // 3 = contract_fwd_mx16g<2> (N = 128, group interleave 1); inverse 0 = contract_inv_mx32<4>,
-// 1 = contract_inv_mx32g<2>
+// 1 = contract_inv_mx32g<2>; split plans 4 = contract_fwd_mxks / contract_inv_mxks
 struct MultiMx {
     MxArgs c[MULTI_MAX_MX];
     int kind[MULTI_MAX_MX];
@@ -128,17 +128,18 @@ __global__ void __launch_bounds__(256) contract_inv_multi(MultiMx m)
     });
 }
 
-// split-BF16 plans (NPC pieces per operand): every class's contraction is the one-group kernel
+// split-BF16 plans (NPC pieces per operand): every class's contraction is the register-operand
+// split kernel of its own launches (contract_fwd_mxks / contract_inv_mxks, 8 waves)
 template <int NPC>
-__global__ void __launch_bounds__(256) contract_fwd_multi_s(MultiMx m)
+__global__ void __launch_bounds__(512) contract_fwd_multi_s(MultiMx m)
 {
-    multi_drain(m.q, [&](int e, int unit) { contract_fwd_mxs_body<4, NPC>(m.c[e], unit); });
+    multi_drain(m.q, [&](int e, int unit) { contract_fwd_mxks_body<4, NPC, 8>(m.c[e], unit); });
 }
 
 template <int NPC>
-__global__ void __launch_bounds__(256) contract_inv_multi_s(MultiMx m)
+__global__ void __launch_bounds__(512) contract_inv_multi_s(MultiMx m)
 {
-    multi_drain(m.q, [&](int e, int unit) { contract_inv_mxs_body<4, NPC>(m.c[e], unit); });
+    multi_drain(m.q, [&](int e, int unit) { contract_inv_mxks_body<4, NPC, 8>(m.c[e], unit); });
 }
 
 // coefficient layout passes (PT <-> packed complex) of every class

@@ -71,8 +71,12 @@ enum fftso3_precision {
  * (each a multiple of 8, up to l_max = 63) runs each pipeline stage as one persistent launch over
  * all its bandlimit classes: workgroups claim (class, unit) items, largest bandlimit first, from
  * per-XCD queues with atomic counters; FFTSO3_FLAG_PER_CLASS_LAUNCHES asks for one launch per
- * class and stage instead (the same results to 1e-6 relative). */
+ * class and stage instead (the same results to 1e-6 relative); FFTSO3_FLAG_PER_CLASS_CONTRACTIONS
+ * keeps the single launches of the FFT and layout stages but launches each class's contractions
+ * on their own (the same results as per-class launches).  fftso3_plan_autotune chooses among the
+ * three on its own. */
 #define FFTSO3_FLAG_PER_CLASS_LAUNCHES 1u
+#define FFTSO3_FLAG_PER_CLASS_CONTRACTIONS 2u
 
 /* Uniform batch: n_fn functions of bandlimit B = l_max + 1.  flags: 0 or the flags above. */
 int fftso3_plan_create(fftso3_plan_t *plan, int domain, int l_max, int n_fn, int kind, int precision,

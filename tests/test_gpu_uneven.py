@@ -62,10 +62,11 @@ def test_unified_matches_per_class_and_oracle(fftso3, torch_cuda, oracle, kind, 
           for i, B in enumerate(Bs)]
     flat = np.concatenate(Fs)
     outs = {}
-    for flags in (0, fftso3.FLAG_PER_CLASS_LAUNCHES):
+    for flags in (0, fftso3.FLAG_PER_CLASS_LAUNCHES, fftso3.FLAG_PER_CLASS_CONTRACTIONS):
         plan = fftso3.Plan([B - 1 for B in Bs], domain="so3", kind=kind, precision=precision, flags=flags)
         desc = plan.describe()
-        assert ("unified" in desc) == (flags == 0), desc
+        assert ("unified" in desc) == (flags != fftso3.FLAG_PER_CLASS_LAUNCHES), desc
+        assert ("contractions per class" in desc) == (flags == fftso3.FLAG_PER_CLASS_CONTRACTIONS), desc
         gdt, cdt = plan.dtypes()
         dF = torch.from_numpy(flat).to(cdt).cuda()
         plan.timing(True)
@@ -76,11 +77,24 @@ def test_unified_matches_per_class_and_oracle(fftso3, torch_cuda, oracle, kind, 
         plan.timing(False)
         if flags == 0:
             assert all(v[1] == 1 for k, v in st.items() if k in ("fft_fwd", "contract_fwd", "contract_inv", "fft_inv"))
+        if flags == fftso3.FLAG_PER_CLASS_CONTRACTIONS:
+            assert all(v[1] == 1 for k, v in st.items() if k in ("fft_fwd", "fft_inv"))
         outs[flags] = (g.cpu().numpy(), c.cpu().numpy(), [plan.grid_offset(i) for i in range(len(Bs))],
                        [plan.coef_offset(i) for i in range(len(Bs))])
         plan.close()
     g0, c0, go, co = outs[0]
     g1, c1, _, _ = outs[fftso3.FLAG_PER_CLASS_LAUNCHES]
+    g2, c2, go2, co2 = outs[fftso3.FLAG_PER_CLASS_CONTRACTIONS]
+    # unified FFT stages with per-class contractions: the per-class results (the same kernels at
+    # every stage but the FFTs, whose unified and per-class forms agree bit for bit below l_max = 63)
+    for i, B in enumerate(Bs):
+        gs = slice(go[i], go[i] + (2 * B) ** 3)
+        cs = slice(co[i], co[i] + len(Fs[i]))
+        assert go2[i] == go[i] and co2[i] == co[i]
+        if B < 64:
+            assert np.array_equal(g1[gs], g2[gs]) and np.array_equal(c1[cs], c2[cs]), (i, B)
+        else:
+            assert rel_l2(g1[gs], g2[gs]) < 1e-6 and rel_l2(c1[cs], c2[cs]) < 1e-6, (i, B)
     for i, B in enumerate(Bs):
         gs = slice(go[i], go[i] + (2 * B) ** 3)
         cs = slice(co[i], co[i] + len(Fs[i]))
