@@ -850,7 +850,7 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
             const int fbw = 16 / ag.gpw, nfbg = (ca.nfc + fbw - 1) / fbw;
             ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
             const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
-            const size_t glds = (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb;
+            const size_t glds = mx16g_lds_bytes(c->B);
             if (ag.gpw == 2) {
                 // groups with at most 32 degrees (m0 >= B - 32; 3/4 of them at l_max = 63) keep
                 // two 16-row tiles of D in registers instead of four: fewer registers, two
@@ -867,7 +867,8 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
                     hipLaunchKernelGGL((contract_fwd_mx16g<2>), dim3(deal_grid(a2.nitems * a2.nsplit, a2.nsplit)),
                                        dim3(256), glds, s, a2);
             }
-            else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256), glds, s, ag);
+            else hipLaunchKernelGGL((contract_fwd_mx32g<4>), dim3(ggrid), dim3(256),
+                                    (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb, s, ag);
         } else if (c->N == 128) hipLaunchKernelGGL((contract_fwd_mx32<NP, true>), dim3(mgrid), dim3(256), lds, s, a);
         else hipLaunchKernelGGL((contract_fwd_mx32<NP, false>), dim3(mgrid), dim3(256), lds, s, a);
     } else {
@@ -898,7 +899,7 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
             ag.nitems = c->nitems_mx / 2;
             const int nfbg = (ca.nfc + 7) / 8;
             ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
-            const size_t glds = (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
+            const size_t glds = mx32g_inv_lds_bytes(c->B);
             hipLaunchKernelGGL((contract_inv_mx32g<2>), dim3(deal_grid(ag.nitems * ag.nsplit, ag.nsplit)), dim3(256),
                                glds, s, ag);
         } else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
@@ -1531,8 +1532,7 @@ int run_unified(fftso3_plan_s *pl, bool fwd, const void *in, void *out, hipStrea
             const size_t rcb = (size_t)c->B * sizeof(RecCoef);
             const size_t l = npc ? (f ? mxks_fwd_lds_bytes(c->N, npc, 4) : mxks_inv_lds_bytes(npc, 4, c->B)) + rcb
                            : ent[e].second == 0 ? MX_LDS_BYTES + rcb
-                           : f ? (size_t)128 * MX_LPITCH * sizeof(float) + 2 * rcb
-                               : (size_t)(2 * 64 * MX_DPITCH + 128 * MX_LPITCH) * sizeof(float) + 2 * rcb;
+                           : f ? mx16g_lds_bytes(c->B) : mx32g_inv_lds_bytes(c->B);
             lds = std::max(lds, l);
         }
         m.q.items = reinterpret_cast<const int2 *>((f ? pl->d_q_cf : pl->d_q_ci).p);

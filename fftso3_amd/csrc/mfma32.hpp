@@ -28,6 +28,23 @@ typedef float f32x16m_t __attribute__((ext_vector_type(16)));
 #define MX_DPITCH 129        // LDS pitch of D rows (floats): conflict-free column reads
 #define MX_LPITCH 132        // LDS pitch of spectrum-block rows (floats)
 #define MX_LDS_BYTES ((64 * MX_DPITCH + 128 * MX_LPITCH) * 4)
+// Block and d rows of the grouped kernels (contract_fwd_mx16g, contract_inv_mx32g) are 128
+// floats long, with columns XOR-swizzled by row bits so gfx950's 64 LDS banks see no conflict (a
+// 128-float pitch also keeps every operand offset within the 16-bit LDS immediate):
+//   forward block: column c of row r at c ^ 16 (r & 3) -- the 4 rows of a 16 x 16 x 4 operand
+//     read, and the 2 rows of one 8-byte staging access, take disjoint 16-bank ranges;
+//   inverse d: column k of row r at k ^ 32 (r & 1) -- the 2 rows of a 32 x 32 x 2 operand read;
+//   inverse output block: column c of row k at c ^ 32 ((k >> 2) & 1) ^ 16 (k & 1) -- the
+//     accumulator writes (lane halves 4 rows apart) and the 8-byte reads of two adjacent rows.
+#define MXG_LPITCH 128
+#define MXG_DPITCH 128
+// the forward's block region first holds the two groups' D staging ([2][64][MX_DPITCH])
+#define MX16G_BLOCK_BYTES (2 * 64 * MX_DPITCH * 4 > 128 * MXG_LPITCH * 4 ? 2 * 64 * MX_DPITCH * 4 : 128 * MXG_LPITCH * 4)
+__host__ __device__ inline size_t mx16g_lds_bytes(int B) { return (size_t)MX16G_BLOCK_BYTES + 2 * (size_t)B * sizeof(RecCoef); }
+__host__ __device__ inline size_t mx32g_inv_lds_bytes(int B)
+{
+    return (size_t)(2 * 64 * MXG_DPITCH + 128 * MXG_LPITCH) * 4 + 2 * (size_t)B * sizeof(RecCoef);
+}
 
 struct MxArgs {
     ChunkArgs ca;            // ca.C = functions of the class (workspace stride)
@@ -412,9 +429,9 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
 {
     constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float *L = reinterpret_cast<float *>(smem_raw);     // [128][MX_LPITCH], spectrum block
+    float *L = reinterpret_cast<float *>(smem_raw);     // [128][MXG_LPITCH], spectrum block
     float *Dst = L;                                      // [2][64][MX_DPITCH] staging, before L is used
-    RecCoef *rcs = reinterpret_cast<RecCoef *>(smem_raw + 128 * MX_LPITCH * 4);   // [2][B]
+    RecCoef *rcs = reinterpret_cast<RecCoef *>(smem_raw + MX16G_BLOCK_BYTES);   // [2][B]
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, N = ca.N, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc + FB - 1) / FB;
@@ -472,14 +489,17 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
     const float sg0 = (float)so3_pair_sign(m0, n0, pg * 2), sg1 = (float)so3_pair_sign(m0, n0, pg * 2 + 1);
     const bool pv0 = so3_pair_valid(m0, n0, pg * 2), pv1 = so3_pair_valid(m0, n0, pg * 2 + 1);
     float *PT = reinterpret_cast<float *>(a.dst);
-    const float *Lw = L + kq * MX_LPITCH + wave * 32 + cx;
+    // operand rows 4 s + kq: columns swizzled by 16 kq
+    const float *Lw0 = L + kq * MXG_LPITCH + ((wave * 32 + cx) ^ (16 * kq));
+    const float *Lw1 = L + kq * MXG_LPITCH + ((wave * 32 + 16 + cx) ^ (16 * kq));
+    const int lsw = 16 * (r0 & 3);
     for (int fb = fb0; fb < fb1; fb++) {
         __syncthreads();                                 // previous block consumed
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            float *row = L + (r0 + 8 * i) * MX_LPITCH;
-            *reinterpret_cast<float2 *>(row + lc0) = make_float2(pf[i].x, pf[i].y);
-            *reinterpret_cast<float2 *>(row + lc1) = make_float2(pf[i].z, pf[i].w);
+            float *row = L + (r0 + 8 * i) * MXG_LPITCH;   // (r0 + 8i) & 3 = r0 & 3
+            *reinterpret_cast<float2 *>(row + (lc0 ^ lsw)) = make_float2(pf[i].x, pf[i].y);
+            *reinterpret_cast<float2 *>(row + (lc1 ^ lsw)) = make_float2(pf[i].z, pf[i].w);
         }
         __syncthreads();
         if (fb + 1 < fb1) load_block(fb + 1);
@@ -497,8 +517,8 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
                 float b[4][2];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    b[u][0] = Lw[4 * (s0 + u) * MX_LPITCH];
-                    b[u][1] = Lw[4 * (s0 + u) * MX_LPITCH + 16];
+                    b[u][0] = Lw0[4 * (s0 + u) * MXG_LPITCH];
+                    b[u][1] = Lw1[4 * (s0 + u) * MXG_LPITCH];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -687,9 +707,11 @@ __device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int uni
     static_assert(GPW == 2, "the two groups of one interleaved workspace line");
     constexpr int NP = 4, FB = 16 / GPW, WPG = 4 / GPW;  // functions per block, waves per group
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    float *D = reinterpret_cast<float *>(smem_raw);     // [GPW][64][MX_DPITCH], rows l - m0, columns k
-    float *O = D + GPW * 64 * MX_DPITCH;                 // [128][MX_LPITCH], output block
-    RecCoef *rcs = reinterpret_cast<RecCoef *>(O + 128 * MX_LPITCH);   // [GPW][B]
+    // D: [GPW][64][MXG_DPITCH], rows l - m0, column k stored at k ^ 32 (l - m0 odd)
+    // O: [128][MXG_LPITCH], output block, column c of row k at c ^ 32 ((k >> 2) & 1) ^ 16 (k & 1)
+    float *D = reinterpret_cast<float *>(smem_raw);
+    float *O = D + GPW * 64 * MXG_DPITCH;
+    RecCoef *rcs = reinterpret_cast<RecCoef *>(O + 128 * MXG_LPITCH);   // [GPW][B]
     const ChunkArgs &ca = a.ca;
     const int B = ca.B, C = ca.C, CW = C * NP * 2;
     const int nfb = (ca.nfc + FB - 1) / FB;
@@ -722,26 +744,29 @@ __device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int uni
         int hm0, hn0;
         so3_group(gb + h, hm0, hn0);
         RecCoef *rc = rcs + h * B;
-        float *Dh = D + h * 64 * MX_DPITCH;
+        float *Dh = D + h * 64 * MXG_DPITCH;
         group_coefs(a.rt, 0, B, hm0, hn0, rc, k, 128);
         __syncthreads();
         const int hM = B - hm0, hM4 = (hM + 3) & ~3;
         group_recursion_lds(a.rt, rc, 0, gb + h, hm0, hn0, k, B, [&](int l, double v) {
-            Dh[(l - hm0) * MX_DPITCH + k] = (float)((double)(2 * l + 1) * v);
+            Dh[(l - hm0) * MXG_DPITCH + (k ^ (((l - hm0) & 1) << 5))] = (float)((double)(2 * l + 1) * v);
         });
-        for (int r = hM; r < hM4; r++) Dh[r * MX_DPITCH + k] = 0.f;
+        for (int r = hM; r < hM4; r++) Dh[r * MXG_DPITCH + (k ^ ((r & 1) << 5))] = 0.f;
     }
     const float sg = (float)so3_pair_sign(m0, n0, p);
     // store role: rows r0 + 8i (i < 16) of the block, columns 4q..4q+3 = function q / 4, group
     // (q / 2) & 1, pairs 2(q & 1) and 2(q & 1) + 1; computed by wave wq, lanes 2fq + {0,1} and 16 + 2fq + {0,1}
     const int q = tid & 31, r0 = tid >> 5;
     const int fq = q >> 2, wq = ((q >> 1) & 1) * WPG + (q & 1);
-    const int lc0 = wq * 32 + 2 * fq, lc1 = lc0 + 16;
+    const int osw = 32 * ((r0 >> 2) & 1) + 16 * (r0 & 1);      // rows r0 + 8i share bits 0 and 2
+    const int lc0 = (wq * 32 + 2 * fq) ^ osw, lc1 = (wq * 32 + 16 + 2 * fq) ^ osw;
     float *G = reinterpret_cast<float *>(a.dst);
     long roff[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) roff[i] = ws_row(ca, gb, r0 + 8 * i) * CW + 4 * q;
-    const float *Dg = D + gw * 64 * MX_DPITCH + kh * MX_DPITCH + cl;
+    // the lane's rows 2s + kh have parity kh: column 32t + cl sits at 32 (t ^ kh) + cl
+    const float *Dg = D + gw * 64 * MXG_DPITCH + kh * MXG_DPITCH + cl;
+    const float *DgE = Dg + 32 * kh, *DgO = Dg - 32 * kh;   // even / odd t
     auto multiply_store = [&](const float *buf, int fb) {
         f32x16m_t acc[4];
 #pragma unroll
@@ -752,12 +777,11 @@ __device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int uni
 #pragma unroll
         for (int s = 0; s < 32; s += 2) {
             if (4 * (s >> 1) < M4) {
-                const float *Dr = Dg + 2 * s * MX_DPITCH;
                 float d[2][4];
 #pragma unroll
                 for (int u = 0; u < 2; u++)
 #pragma unroll
-                    for (int t = 0; t < 4; t++) d[u][t] = Dr[2 * u * MX_DPITCH + 32 * t];
+                    for (int t = 0; t < 4; t++) d[u][t] = (t & 1 ? DgO : DgE)[(2 * s + 2 * u) * MXG_DPITCH + 32 * t];
 #pragma unroll
                 for (int u = 0; u < 2; u++)
 #pragma unroll
@@ -771,13 +795,13 @@ __device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int uni
 #pragma unroll
             for (int j = 0; j < 16; j++) {
                 const int k = t * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);
-                O[k * MX_LPITCH + wave * 32 + cl] = sg * acc[t][j];
+                O[k * MXG_LPITCH + ((wave * 32 + cl) ^ (32 * kh) ^ (16 * (j & 1)))] = sg * acc[t][j];   // k: bit 2 kh, bit 0 j
             }
         __syncthreads();
         const bool ok = fb * FB + fq < ca.nfc;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            const float *row = O + (r0 + 8 * i) * MX_LPITCH;
+            const float *row = O + (r0 + 8 * i) * MXG_LPITCH;
             const float2 u = *reinterpret_cast<const float2 *>(row + lc0);
             const float2 v = *reinterpret_cast<const float2 *>(row + lc1);
             float *d = ok ? G + roff[i] + (long)fb * 128 : a.dummy;
