@@ -930,7 +930,7 @@ __host__ __device__ inline size_t mxk_lds_bytes(int N, int B, int NP, bool fwd)
     const int N32 = (N + 31) & ~31;
     const size_t table = (size_t)NP * MXK_MAXROWS * 4;
     if (fwd) return (size_t)32 * ((B + 31) / 32) * (N32 + 1) * 4 + table;
-    return (size_t)((B + 3) & ~3) * (N + 1) * 4 + table;
+    return (size_t)((B + 31) & ~31) * (N + 1) * 4 + table;
 }
 
 template <int NP, int NW>
@@ -1035,8 +1035,9 @@ __global__ void __launch_bounds__(512) contract_fwd_mxk(MxArgs a)
 }
 
 
-// Inverse with the group's c_l d^l block resident in LDS ([M4][N+1], rows l - m0 zero-padded to a
-// multiple of 4); wave w produces block columns [32w, 32w+32) for every beta row.
+// Inverse with the group's c_l d^l block resident in LDS ([M32][N+1], rows l - m0 zero-padded to a
+// multiple of 32, so each group of 16 MFMA steps runs whole); wave w produces block columns
+// [32w, 32w+32) for every beta row.
 // NW waves per workgroup, each owning 32 columns of a block: 8 waves (2 per SIMD) share the
 // group's d block so one wave's MFMAs cover another's LDS and HBM waits.
 template <int NP, int NW>
@@ -1050,9 +1051,9 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
     if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
-    const int M = B - m0, M4 = (M + 3) & ~3, DP = N + 1, nkt = (N + 31) / 32;
-    float *D = reinterpret_cast<float *>(smem_raw);                 // [M4][DP]
-    int *etab = reinterpret_cast<int *>(D + (size_t)M4 * DP);       // [NP][MXK_MAXROWS]
+    const int M = B - m0, M32 = (M + 31) & ~31, DP = N + 1, nkt = (N + 31) / 32;
+    float *D = reinterpret_cast<float *>(smem_raw);                 // [M32][DP], rows >= M zero
+    int *etab = reinterpret_cast<int *>(D + (size_t)M32 * DP);      // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     for (int i = tid; i < NP * MXK_MAXROWS; i += 64 * NW) {
@@ -1067,9 +1068,9 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
             group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * DP + k] = (float)(NP == 2 ? v : (double)(2 * l + 1) * v);
             });
-            for (int r = M; r < M4; r++) D[r * DP + k] = 0.f;
+            for (int r = M; r < M32; r++) D[r * DP + k] = 0.f;
         } else if (k < DP) {
-            for (int r = 0; r < M4; r++) D[r * DP + k] = 0.f;
+            for (int r = 0; r < M32; r++) D[r * DP + k] = 0.f;
         }
     }
     __syncthreads();
@@ -1078,13 +1079,25 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
     const float *PT = reinterpret_cast<const float *>(a.src);
     float *G = reinterpret_cast<float *>(a.dst);
     const int *et = etab + p * MXK_MAXROWS;
-    for (int fb = fb0; fb < fb1; fb++) {
+    // coefficient rows l = m0 + 2s + kh of block fb's column, 16 steps (32 degrees) at a time up to
+    // M; rows past M - 1 read row M - 1 (finite values against the zero rows of D).  The next
+    // block's are loaded while the current block multiplies.
+    auto load = [&](float *x, int fb) {
         const int c = fb * 32 * NW + cb, cv = c < NCOL ? c : 0;
         const float *src = PT + 2 * (cv / (2 * NP)) + (cv & 1);
-        // coefficient rows l = m0 + 2s + kh (clamped to finite rows; zero rows of D beyond M)
-        float buf[64];
+        int kb = kh;   // the row offsets are read from the table at each load, not held across blocks
+        asm volatile("" : "+v"(kb));
 #pragma unroll
-        for (int s = 0; s < 64; s++) buf[s] = 2 * s < M4 ? src[2L * et[2 * s + kh] * C] : 0.f;
+        for (int s0 = 0; s0 < 64; s0 += 16)
+            if (2 * s0 < M)
+#pragma unroll
+                for (int u = 0; u < 16; u++) x[s0 + u] = src[2L * et[2 * (s0 + u) + kb] * C];
+    };
+    float buf[64] = {}, nbuf[64] = {};   // steps past the group's last 16-step group stay unused
+    load(buf, fb0);
+    for (int fb = fb0; fb < fb1; fb++) {
+        const int c = fb * 32 * NW + cb;
+        if (fb + 1 < fb1) load(nbuf, fb + 1);
         for (int t = 0; t < nkt; t++) {
             f32x16m_t acc;
 #pragma unroll
@@ -1096,24 +1109,28 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
             // 16 operand reads from LDS ahead of their 16 MFMAs, so no MFMA waits on its own read
 #pragma unroll
             for (int s0 = 0; s0 < 64; s0 += 16) {
-                if (2 * s0 < M4) {
+                if (2 * s0 < M) {
                     float d[16];
 #pragma unroll
-                    for (int u = 0; u < 16; u++) d[u] = Dt[min(2 * (s0 + u), M4 - 2) * DP];   // rows past M4: unused
+                    for (int u = 0; u < 16; u++) d[u] = Dt[2 * (s0 + u) * DP];
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int u = 0; u < 16; u++)
-                        if (2 * (s0 + u) < M4) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], buf[s0 + u], acc, 0, 0, 0);
+                    for (int u = 0; u < 16; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], buf[s0 + u], acc, 0, 0, 0);
                 }
             }
             const bool ok = c < NCOL;
+            // the row offsets are formed at the stores rather than held across the block loop
+            int kt = 32 * t + 4 * kh;
+            asm volatile("" : "+v"(kt));
 #pragma unroll
             for (int j = 0; j < 16; j++) {
-                const int k = 32 * t + 8 * (j >> 2) + 4 * kh + (j & 3);
+                const int k = kt + 8 * (j >> 2) + (j & 3);
                 float *d = ok && k < N ? G + ws_row(ca, g, k) * CW + c : a.dummy;
                 *d = sg * acc[j];
             }
         }
+#pragma unroll
+        for (int s = 0; s < 64; s++) buf[s] = nbuf[s];
     }
 }
 
