@@ -803,6 +803,7 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
                        hipStream_t s)
 {
     MxArgs a;
+    a.mrows = 0;
     a.ca = ca;
     a.rt = rec_tables(c);
     a.items = (const int *)c->d_items_mx.p;
@@ -835,12 +836,23 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
                 hipLaunchKernelGGL((contract_fwd_mxks<NP, 2>), dim3(fgrid), dim3(512),
                                    mxks_fwd_lds_bytes(c->N, 2, NP) + rcb, s, af);
         } else if (c->mxk) {
-            // 8 waves per workgroup over blocks of 256 columns: half as many units
-            MxArgs a8 = a;
+            // 8 waves per workgroup over blocks of 256 columns: half as many units.  The groups of
+            // at most 64 degrees (items come largest first) run in a second launch whose LDS holds
+            // 64 d rows, so two workgroups share a CU.
+            const int nbig = c->B <= 64 ? a.nitems : NP == 2 ? c->B - 64 : (c->B - 64) * (c->B - 64);
             const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
-            a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
-            hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
-                               dim3(512), mxk_lds_bytes(c->N, c->B, NP, true) + rcb, s, a8);
+            for (int part = 0; part < 2; part++) {
+                MxArgs a8 = a;
+                if (part == 1) {
+                    a8.items += nbig;
+                    a8.mrows = 64;
+                }
+                a8.nitems = part == 0 ? nbig : a.nitems - nbig;
+                if (a8.nitems <= 0) continue;
+                a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a8.nitems - 1) / a8.nitems)));
+                hipLaunchKernelGGL((contract_fwd_mxk<NP>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
+                                   dim3(512), mxk_lds_bytes(c->N, part ? 64 : c->B, NP, true) + rcb, s, a8);
+            }
         }
         else if (c->N == 128 && NP == 4 && ca.gil > 0 && ca.gil <= 2) {
             // the 2^gil groups that share workspace lines in one workgroup
@@ -1323,6 +1335,7 @@ bool unified_eligible(const fftso3_plan_s *pl)
 MxArgs make_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, int kind)
 {
     MxArgs a;
+    a.mrows = 0;
     a.ca = ca;
     a.rt = rec_tables(c);
     a.items = (const int *)c->d_items_mx.p;

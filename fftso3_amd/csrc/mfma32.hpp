@@ -57,6 +57,7 @@ struct MxArgs {
     const void *src;         // forward: spectrum workspace; inverse: PT
     void *dst;               // forward: PT; inverse: spectrum workspace
     float *dummy;            // 16-byte slot that absorbs the stores of lanes with nothing to store
+    int mrows;               // K-chunked forward: degree rows the launch's LDS holds (0: the bandlimit)
 };
 
 // Work unit of a batch-wide kernel: group a.items[u / nsplit] and its column blocks [fb0, fb1)
@@ -950,7 +951,7 @@ __device__ __forceinline__ void contract_fwd_mxk_body(const MxArgs &a, int unit)
     int *etab = reinterpret_cast<int *>(D + (size_t)32 * nrt * DP); // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
-    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, B, NP, true));
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, a.mrows ? a.mrows : B, NP, true));
     group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 64 * NW);
     __syncthreads();
     for (int k = tid; k < N32; k += 64 * NW) {
