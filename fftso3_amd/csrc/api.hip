@@ -895,14 +895,27 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
                 hipLaunchKernelGGL((contract_inv_mxks<NP, 3>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(3, NP, c->B) + rcb, s, ai);
             else
                 hipLaunchKernelGGL((contract_inv_mxks<NP, 2>), dim3(igrid), dim3(512), mxks_inv_lds_bytes(2, NP, c->B) + rcb, s, ai);
-        } else if (c->mxk)
-        {
-            // 8 waves per workgroup over blocks of 256 columns: half as many units
-            MxArgs a8 = a;
+        } else if (c->mxk) {
+            // 8 waves per workgroup over blocks of 256 columns: half as many units.  The groups of
+            // at most 64 degrees run in a second launch (contract_inv_mxk64: a 64-row d block, two
+            // workgroups per CU).
+            const int nbig = c->B <= 64 ? a.nitems : NP == 2 ? c->B - 64 : (c->B - 64) * (c->B - 64);
             const int nfb8 = (ca.nfc * NP * 2 + 255) / 256;
-            a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a.nitems - 1) / a.nitems)));
-            hipLaunchKernelGGL((contract_inv_mxk<NP, 8>), dim3(deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil)),
-                               dim3(512), mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a8);
+            for (int part = 0; part < 2; part++) {
+                MxArgs a8 = a;
+                if (part == 1) {
+                    a8.items += nbig;
+                    a8.mrows = 64;
+                }
+                a8.nitems = part == 0 ? nbig : a.nitems - nbig;
+                if (a8.nitems <= 0) continue;
+                a8.nsplit = std::max(1, std::min(nfb8, pl->split_mul * ((1024 + a8.nitems - 1) / a8.nitems)));
+                const int grid = deal_grid(a8.nitems * a8.nsplit, a8.nsplit << ca.gil);
+                if (part == 0)
+                    hipLaunchKernelGGL((contract_inv_mxk<NP, 8>), dim3(grid), dim3(512), mxk_lds_bytes(c->N, c->B, NP, false) + rcb, s, a8);
+                else
+                    hipLaunchKernelGGL((contract_inv_mxk64<NP>), dim3(grid), dim3(512), mxk_lds_bytes(c->N, 64, NP, false) + rcb, s, a8);
+            }
         }
         else if (c->N == 128 && NP == 4 && ca.gil == 1) {
             // the two groups that share workspace lines in one workgroup
@@ -1721,6 +1734,8 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mxk<2>));
     ALLOW_LDS((contract_inv_mxk<4, 8>));
     ALLOW_LDS((contract_inv_mxk<2, 8>));
+    ALLOW_LDS((contract_inv_mxk64<4>));
+    ALLOW_LDS((contract_inv_mxk64<2>));
     ALLOW_LDS((contract_fwd_mxks<4, 2>));
     ALLOW_LDS((contract_fwd_mxks<4, 3>));
     ALLOW_LDS((contract_fwd_mxks<2, 2>));

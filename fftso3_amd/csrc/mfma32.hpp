@@ -1041,7 +1041,7 @@ __global__ void __launch_bounds__(512) contract_fwd_mxk(MxArgs a)
 // [32w, 32w+32) for every beta row.
 // NW waves per workgroup, each owning 32 columns of a block: 8 waves (2 per SIMD) share the
 // group's d block so one wave's MFMAs cover another's LDS and HBM waits.
-template <int NP, int NW>
+template <int NP, int NW, bool PF = true>
 __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1061,7 +1061,7 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
         const int p = i / MXK_MAXROWS, r = i - p * MXK_MAXROWS;
         etab[i] = MxPair<NP>(m0, n0, p).packed(m0 + min(r, M - 1));
     }
-    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, B, NP, false));
+    RecCoef *rc = reinterpret_cast<RecCoef *>(smem_raw + mxk_lds_bytes(N, a.mrows ? a.mrows : B, NP, false));
     group_coefs(a.rt, NP == 2, B, m0, n0, rc, tid, 64 * NW);
     __syncthreads();
     for (int k = tid; k < 32 * nkt; k += 64 * NW) {
@@ -1095,10 +1095,11 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
                 for (int u = 0; u < 16; u++) x[s0 + u] = src[2L * et[2 * (s0 + u) + kb] * C];
     };
     float buf[64] = {}, nbuf[64] = {};   // steps past the group's last 16-step group stay unused
-    load(buf, fb0);
+    if (PF) load(buf, fb0);
     for (int fb = fb0; fb < fb1; fb++) {
         const int c = fb * 32 * NW + cb;
-        if (fb + 1 < fb1) load(nbuf, fb + 1);
+        if (!PF) load(buf, fb);
+        else if (fb + 1 < fb1) load(nbuf, fb + 1);
         for (int t = 0; t < nkt; t++) {
             f32x16m_t acc;
 #pragma unroll
@@ -1130,8 +1131,9 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
                 *d = sg * acc[j];
             }
         }
+        if (PF)
 #pragma unroll
-        for (int s = 0; s < 64; s++) buf[s] = nbuf[s];
+            for (int s = 0; s < 64; s++) buf[s] = nbuf[s];
     }
 }
 
@@ -1139,6 +1141,14 @@ template <int NP, int NW>
 __global__ void __launch_bounds__(64 * NW) contract_inv_mxk(MxArgs a)
 {
     contract_inv_mxk_body<NP, NW>(a, mx_block_unit(a));
+}
+
+// the groups of at most 64 degrees: a 64-row d block and no operand prefetch, so that two
+// workgroups share a CU
+template <int NP>
+__global__ void __launch_bounds__(512, 2) contract_inv_mxk64(MxArgs a)
+{
+    contract_inv_mxk_body<NP, 8, false>(a, mx_block_unit(a));
 }
 
 // ---------------------------------------------------------------------------------------------
