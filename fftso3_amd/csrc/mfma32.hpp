@@ -425,7 +425,7 @@ typedef float f32x4m_t __attribute__((ext_vector_type(4)));
 // (44% of the groups at l_max = 63).  Tile layouts (lane i): A[i%16][i/16] = D(l, k), B[i/16][i%16]
 // = spectrum(k, column), result row 4(i/16) + j, column i%16.  Wave w takes group w / 2 and pairs
 // 2(w % 2), 2(w % 2) + 1: one 16-column tile per pair (8 functions, re/im).
-template <int NRT>
+template <int NRT, bool PAIR = true>
 __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int unit)
 {
     constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2;
@@ -491,6 +491,7 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
     const bool pv0 = so3_pair_valid(m0, n0, pg * 2), pv1 = so3_pair_valid(m0, n0, pg * 2 + 1);
     float *PT = reinterpret_cast<float *>(a.dst);
     // operand rows 4 s + kq: columns swizzled by 16 kq
+    f32x4m_t accp[NRT][2];   // results of the first block of a pair
     const float *Lw0 = L + kq * MXG_LPITCH + ((wave * 32 + cx) ^ (16 * kq));
     const float *Lw1 = L + kq * MXG_LPITCH + ((wave * 32 + 16 + cx) ^ (16 * kq));
     const int lsw = 16 * (r0 & 3);
@@ -535,6 +536,18 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
         else if (NRT == 2 || nrt == 2) run(std::integral_constant<int, 2>());
         else if (nrt == 3) run(std::integral_constant<int, NRT >= 3 ? 3 : 2>());
         else run(std::integral_constant<int, NRT>());
+        // blocks 2i and 2i + 1 fill the two 64-byte halves of each 128-byte line of PT (16
+        // functions): the first block's results wait in registers so both halves are stored by
+        // back-to-back instructions
+        const bool first = PAIR && !(fb & 1) && fb + 1 < fb1;
+        if (first) {
+#pragma unroll
+            for (int t = 0; t < NRT; t++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) accp[t][c] = acc[t][c];
+            continue;
+        }
+        const bool paired = PAIR && (fb & 1) && fb > fb0;
         const int fn = fb * FB + fo;
         const bool okf = fn < ca.nfc;
         float *o = PT + 2 * fn + (cx & 1);
@@ -545,8 +558,15 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
                 for (int j = 0; j < 4; j++) {
                     const int r = 16 * t + 4 * kq + j;
                     const long e0 = pr0.packed(m0 + min(r, M - 1)), e1 = pr1.packed(m0 + min(r, M - 1));
-                    float *d0 = okf && pv0 && r < M ? o + 2 * e0 * C : a.dummy;
-                    float *d1 = okf && pv1 && r < M ? o + 2 * e1 * C : a.dummy;
+                    const bool ok0 = pv0 && r < M, ok1 = pv1 && r < M;
+                    if (paired) {   // the previous block: functions 8 lower, always < nfc
+                        float *p0 = ok0 ? o - 2 * FB + 2 * e0 * C : a.dummy;
+                        float *p1 = ok1 ? o - 2 * FB + 2 * e1 * C : a.dummy;
+                        *p0 = sg0 * accp[t][0][j];
+                        *p1 = sg1 * accp[t][1][j];
+                    }
+                    float *d0 = okf && ok0 ? o + 2 * e0 * C : a.dummy;
+                    float *d1 = okf && ok1 ? o + 2 * e1 * C : a.dummy;
                     *d0 = sg0 * acc[t][0][j];
                     *d1 = sg1 * acc[t][1][j];
                 }

@@ -112,8 +112,9 @@ __global__ void __launch_bounds__(256) contract_fwd_multi(MultiMx m)
     multi_drain(m.q, [&](int e, int unit) {
         const MxArgs &a = m.c[e];
         switch (m.kind[e]) {
-        case 2: contract_fwd_mx16g_body<4>(a, unit); break;
-        case 3: contract_fwd_mx16g_body<2>(a, unit); break;
+        // without the paired-block stores, whose extra registers would make this combined kernel spill
+        case 2: contract_fwd_mx16g_body<4, false>(a, unit); break;
+        case 3: contract_fwd_mx16g_body<2, false>(a, unit); break;
         default: contract_fwd_mx32_body<4, false>(a, unit);
         }
     });
