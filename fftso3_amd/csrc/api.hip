@@ -1062,12 +1062,13 @@ int contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const v
 template <int R>
 int launch_r16_R(bool fwd, bool real, const FftArgs<float> &a, hipStream_t s)
 {
-    const int grid = xcd_grid(r16_units<R>(a.ca.nfc));
-    const size_t lds = r16_lds_bytes<R>();
-    if (fwd && real) hipLaunchKernelGGL((so3_fft_r16<R, true, true>), dim3(grid), dim3(1024), lds, s, a);
-    else if (fwd) hipLaunchKernelGGL((so3_fft_r16<R, true, false>), dim3(grid), dim3(1024), lds, s, a);
-    else if (real) hipLaunchKernelGGL((so3_fft_r16<R, false, true>), dim3(grid), dim3(1024), lds, s, a);
-    else hipLaunchKernelGGL((so3_fft_r16<R, false, false>), dim3(grid), dim3(1024), lds, s, a);
+    constexpr int nt = r16_threads<R>();
+    const int grid = xcd_grid(r16_units<R, nt>(a.ca.nfc));
+    const size_t lds = r16_lds_bytes<R, nt>();
+    if (fwd && real) hipLaunchKernelGGL((so3_fft_r16<R, true, true>), dim3(grid), dim3(nt), lds, s, a);
+    else if (fwd) hipLaunchKernelGGL((so3_fft_r16<R, true, false>), dim3(grid), dim3(nt), lds, s, a);
+    else if (real) hipLaunchKernelGGL((so3_fft_r16<R, false, true>), dim3(grid), dim3(nt), lds, s, a);
+    else hipLaunchKernelGGL((so3_fft_r16<R, false, false>), dim3(grid), dim3(nt), lds, s, a);
     HIP_TRY(hipGetLastError());
     return FFTSO3_OK;
 }
@@ -2139,8 +2140,13 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         // the chunked kernels (8 functions per pass)
         for (int sp : {1, 2, 4})
             for (int gil : {0, 1}) cands.push_back({false, 8, 0, sp, gil});
+        // several bandlimit classes: also one launch per class and stage, and unified FFT stages
+        // with per-class contraction launches
         if (plan->classes.size() > 1)
-            for (int sp : {1, 2}) cands.push_back({false, 8, 0, sp, 1, true, true});
+            for (int sp : {1, 2}) {
+                cands.push_back({false, 8, 0, sp, 1, false});
+                cands.push_back({false, 8, 0, sp, 1, true, true});
+            }
         cands.push_back({true, 8, 0, 1, 0});
     }
     if (cands.empty()) return FFTSO3_OK;

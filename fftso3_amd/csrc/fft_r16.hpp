@@ -9,25 +9,31 @@
 // in LDS with row pitch N + 1, so every stage's lane pattern is conflict free.
 //
 // T = N^2 / 16 threads work on one slice (stage B: one 16-point DFT per thread), and a workgroup
-// of 1024 threads works on 1024 / T slices side by side, each thread group in its own LDS region,
+// of NTHR threads works on NTHR / T slices side by side, each thread group in its own LDS region,
 // so the small slices of low bandlimits keep the whole workgroup busy.  A thread group without a
-// slice of its own (the last, partial group, or slices past the class's end) runs the same
-// barriers and LDS work on its own region and skips every global load and store.
+// slice of its own (slices past the class's end, or a partial last group when NTHR is not a
+// multiple of T) runs the same barriers and LDS work on its own region and skips every global
+// load and store.  The class's own launches use whole thread groups only (r16_threads: 576
+// threads and one 74 KB region at R = 6, so two workgroups share a CU); the unified multi-class
+// FFT kernel runs 1024 threads for every R.
 #pragma once
 
 #include "fft128.hpp"
 
-template <int R> struct R16 {
+template <int R, int NTHR = 1024> struct R16 {
     static constexpr int N = 16 * R, P = N + 1, T = N * N / 16;
-    static constexpr int SUB = 1024 / T;                      // whole thread groups per workgroup
-    static constexpr int REGIONS = (1024 + T - 1) / T;        // LDS regions (partial group included)
+    static constexpr int SUB = NTHR / T;                      // whole thread groups per workgroup
+    static constexpr int REGIONS = (NTHR + T - 1) / T;        // LDS regions (partial group included)
     static constexpr size_t SLICE_BYTES = (size_t)N * P * sizeof(float2);
 };
 
-// LDS of a 1024-thread workgroup running R16<R> slices: the regions plus the twiddle table W_N^t
-template <int R> __host__ __device__ constexpr size_t r16_lds_bytes()
+// threads of a class's own R16 launch: the whole thread groups that fit 1024
+template <int R> __host__ __device__ constexpr int r16_threads() { return (1024 / R16<R>::T) * R16<R>::T; }
+
+// LDS of an NTHR-thread workgroup running R16 slices: the regions plus the twiddle table W_N^t
+template <int R, int NTHR = 1024> __host__ __device__ constexpr size_t r16_lds_bytes()
 {
-    return R16<R>::REGIONS * R16<R>::SLICE_BYTES + (size_t)R16<R>::N * sizeof(float2);
+    return R16<R, NTHR>::REGIONS * R16<R, NTHR>::SLICE_BYTES + (size_t)R16<R>::N * sizeof(float2);
 }
 
 // one pass (rows or columns) of N-point DFTs over the slice at buf
@@ -157,7 +163,7 @@ __device__ __forceinline__ void r16_inv_slice(const FftArgs<float> &a, int u, bo
 }
 
 // the thread group, region and slice of thread tid for an item whose first slice is u0
-template <int R>
+template <int R, int NTHR>
 __device__ __forceinline__ void r16_role(int tid, int u0, int nslice, float2 *regions, float2 *&buf, int &tl, int &u,
                                          bool &active)
 {
@@ -165,17 +171,20 @@ __device__ __forceinline__ void r16_role(int tid, int u0, int nslice, float2 *re
     tl = tid - sub * R16<R>::T;
     buf = regions + (size_t)sub * (R16<R>::SLICE_BYTES / sizeof(float2));
     u = u0 + sub;
-    active = sub < R16<R>::SUB && u < nslice;
+    active = sub < R16<R, NTHR>::SUB && u < nslice;
 }
 
-// One class's slices through R16 workgroups: item i covers slices [i SUB, (i+1) SUB).
-template <int R> __host__ __device__ inline int r16_units(int nfc) { return (nfc * R16<R>::N + R16<R>::SUB - 1) / R16<R>::SUB; }
+// One class's slices through NTHR-thread R16 workgroups: item i covers slices [i SUB, (i+1) SUB).
+template <int R, int NTHR = 1024> __host__ __device__ inline int r16_units(int nfc)
+{
+    return (nfc * R16<R>::N + R16<R, NTHR>::SUB - 1) / R16<R, NTHR>::SUB;
+}
 
-template <int R, bool FWD, bool REAL>
+template <int R, bool FWD, bool REAL, int NTHR = 1024>
 __device__ __forceinline__ void r16_item(const FftArgs<float> &a, int item, unsigned char *smem)
 {
     float2 *regions = reinterpret_cast<float2 *>(smem);
-    float2 *twl = regions + R16<R>::REGIONS * (R16<R>::SLICE_BYTES / sizeof(float2));
+    float2 *twl = regions + R16<R, NTHR>::REGIONS * (R16<R>::SLICE_BYTES / sizeof(float2));
     // opaque thread index: in a persistent loop over items the compiler must not hoist the
     // per-lane index arithmetic of every R out of the loop (it would hold it all in registers)
     int tid = threadIdx.x;
@@ -185,17 +194,17 @@ __device__ __forceinline__ void r16_item(const FftArgs<float> &a, int item, unsi
     float2 *buf;
     int tl, u;
     bool active;
-    r16_role<R>(tid, item * R16<R>::SUB, a.ca.nfc * R16<R>::N, regions, buf, tl, u, active);
+    r16_role<R, NTHR>(tid, item * R16<R, NTHR>::SUB, a.ca.nfc * R16<R>::N, regions, buf, tl, u, active);
     if (FWD) r16_fwd_slice<R, REAL>(a, u, active, buf, twl, tl);
     else r16_inv_slice<R, REAL>(a, u, active, buf, twl, tl);
 }
 
 template <int R, bool FWD, bool REAL>
-__global__ void __launch_bounds__(1024) so3_fft_r16(FftArgs<float> a)
+__global__ void __launch_bounds__(r16_threads<R>()) so3_fft_r16(FftArgs<float> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int n = r16_units<R>(a.ca.nfc);
+    const int n = r16_units<R, r16_threads<R>()>(a.ca.nfc);
     const int item = xcd_unit(n);
     if (item >= n) return;
-    r16_item<R, FWD, REAL>(a, item, smem_raw);
+    r16_item<R, FWD, REAL, r16_threads<R>()>(a, item, smem_raw);
 }
