@@ -441,6 +441,45 @@ __device__ __forceinline__ void so3_fft128_fwd2_body(const FftArgs<float> &a, in
     if (OPAQUE) asm volatile("" : "+v"(tid));
     if (tid < F128) twl[tid] = a.tw[tid];
     float2 vA[2][8], vB[2][8];
+    if (REAL && two) {
+        // two real slices in one complex DFT: z = a + i b, Z = X_a + i X_b, and since X(-p) =
+        // conj X(p) for a real slice, X_a(p) = (Z(p) + conj Z(-p)) / 2 and X_b(p) = -i (Z(p) -
+        // conj Z(-p)) / 2.  A record holds the bins p and -p of both its pair couples, so its four
+        // reads of Z give both functions' records.
+        fft128_load<true>(a, k, fi, tid, vA);
+        fft128_load<true>(a, k, fi + 1, tid, vB);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 8; r++) vA[i][r].y = vB[i][r].x;
+        fft128_stageA_lds<true>(vA, buf, twl, tid);
+        fft128_rest(buf, twl, tid);
+        float4 *S4 = reinterpret_cast<float4 *>(a.dst);
+        auto sep = [](float2 z, float2 w, float2 &xa, float2 &xb) {   // w = Z(-p)
+            xa = make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y));
+            xb = make_float2(0.5f * (z.y + w.y), 0.5f * (w.x - z.x));
+        };
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int g = tid + i * 1024;
+            int m0, n0;
+            so3_group(g, m0, n0);
+            const int um = (-m0) & 127, un = (-n0) & 127, vm = m0 & 127, vn = n0 & 127;
+            const float2 p0 = buf[um * P128 + un], p1 = buf[vm * P128 + vn];
+            const float2 p2 = buf[vn * P128 + vm], p3 = buf[un * P128 + um];
+            float2 a0, b0, a1, b1, a2, b2, a3, b3;
+            sep(p0, p1, a0, b0);
+            sep(p1, p0, a1, b1);
+            sep(p2, p3, a2, b2);
+            sep(p3, p2, a3, b3);
+            const long o = ws_rec(ca, g, k, fi) * 2;
+            S4[o] = make_float4(a0.x, a0.y, a1.x, a1.y);
+            S4[o + 1] = make_float4(a2.x, a2.y, a3.x, a3.y);
+            S4[o + 4] = make_float4(b0.x, b0.y, b1.x, b1.y);
+            S4[o + 5] = make_float4(b2.x, b2.y, b3.x, b3.y);
+        }
+        return;
+    }
     fft128_load<REAL>(a, k, fi, tid, vA);
     fft128_stageA_lds<REAL>(vA, buf, twl, tid);
     // the second slice's loads go out once the first slice's inputs are in LDS, and are in flight
@@ -490,11 +529,9 @@ __global__ void __launch_bounds__(1024) so3_fft128_fwd2(FftArgs<float> a)
 
 // Inverse with the same pairing: both slices' records are gathered at once (whole lines), then
 // each slice is scattered to bins, transformed and written in turn.
-template <bool REAL>
-__device__ __forceinline__ void fft128_inv_slice(const FftArgs<float> &a, float2 *buf, const float2 *twl,
-                                                 const float4 (&q)[4][2], int k, int fi, int tid)
+// the bins of one slice from its records (4 groups per thread); row and column 64 are zero
+__device__ __forceinline__ void fft128_inv_scatter(float2 *buf, const float4 (&q)[4][2], int tid)
 {
-    const ChunkArgs &ca = a.ca;
     if (tid < F128) {
         buf[64 * P128 + tid] = make_float2(0.f, 0.f);
         buf[tid * P128 + 64] = make_float2(0.f, 0.f);
@@ -514,6 +551,14 @@ __device__ __forceinline__ void fft128_inv_slice(const FftArgs<float> &a, float2
             }
         }
     }
+}
+
+template <bool REAL>
+__device__ __forceinline__ void fft128_inv_slice(const FftArgs<float> &a, float2 *buf, const float2 *twl,
+                                                 const float4 (&q)[4][2], int k, int fi, int tid)
+{
+    const ChunkArgs &ca = a.ca;
+    fft128_inv_scatter(buf, q, tid);
     __syncthreads();
     rows_from_lds(buf, twl, tid);
     __syncthreads();
@@ -554,6 +599,30 @@ __device__ __forceinline__ void so3_fft128_inv2_body(const FftArgs<float> &a, in
         q0[i][1] = G4[o + 1];
         q1[i][0] = G4[o1];
         q1[i][1] = G4[o1 + 1];
+    }
+    if (REAL && two) {
+        // two real slices in one complex DFT: the bins of Z = X_a + i X_b, whose transform has
+        // the grid of function fi in its real part and that of function fi + 1 in its imaginary part
+        float4 z[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                z[i][h] = make_float4(q0[i][h].x - q1[i][h].y, q0[i][h].y + q1[i][h].x, q0[i][h].z - q1[i][h].w,
+                                      q0[i][h].w + q1[i][h].z);
+        fft128_inv_scatter(buf, z, tid);
+        __syncthreads();
+        rows_from_lds(buf, twl, tid);
+        __syncthreads();
+        rows_stageB(buf, tid);
+        __syncthreads();
+        float *outa = reinterpret_cast<float *>(a.dst) + ca.grid_off[ca.fn[fi]] + (long)k * F128 * F128;
+        float *outb = reinterpret_cast<float *>(a.dst) + ca.grid_off[ca.fn[fi + 1]] + (long)k * F128 * F128;
+        cols_pass_out(buf, twl, tid, [&](int row, int col, float2 v) {
+            outa[row * F128 + col] = v.x;
+            outb[row * F128 + col] = v.y;
+        });
+        return;
     }
     fft128_inv_slice<REAL>(a, buf, twl, q0, k, fi, tid);
     if (!two) return;
