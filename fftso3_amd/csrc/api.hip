@@ -1063,7 +1063,7 @@ template <int R>
 int launch_r16_R(bool fwd, bool real, const FftArgs<float> &a, hipStream_t s)
 {
     constexpr int nt = r16_threads<R>();
-    const int grid = xcd_grid(r16_units<R, nt>(a.ca.nfc));
+    const int grid = xcd_grid(r16_units<R, nt>(a.ca.nfc, real));
     const size_t lds = r16_lds_bytes<R, nt>();
     if (fwd && real) hipLaunchKernelGGL((so3_fft_r16<R, true, true>), dim3(grid), dim3(nt), lds, s, a);
     else if (fwd) hipLaunchKernelGGL((so3_fft_r16<R, true, false>), dim3(grid), dim3(nt), lds, s, a);
@@ -1417,15 +1417,17 @@ struct XcdQueue {
     }
 };
 
-int fft_units(const Class *c)
+int fft_units(const fftso3_plan_s *pl, const Class *c)
 {
-    if (c->N == 128) return so3_fft128_fwd2_units((int)c->fns.size());
+    const int nf = (int)c->fns.size();
+    const bool real = pl->kind == FFTSO3_REAL;
+    if (c->N == 128) return so3_fft128_fwd2_units(nf);
     switch (c->r16) {
-    case 1: return r16_units<1>((int)c->fns.size());
-    case 2: return r16_units<2>((int)c->fns.size());
-    case 3: return r16_units<3>((int)c->fns.size());
-    case 4: return r16_units<4>((int)c->fns.size());
-    default: return r16_units<6>((int)c->fns.size());
+    case 1: return r16_units<1>(nf, real);
+    case 2: return r16_units<2>(nf, real);
+    case 3: return r16_units<3>(nf, real);
+    case 4: return r16_units<4>(nf, real);
+    default: return r16_units<6>(nf, real);
     }
 }
 
@@ -1452,7 +1454,7 @@ int build_unified(fftso3_plan_s *pl)
         Class *c = pl->uorder[i];
         const int nfc = (int)c->fns.size();
         const ChunkArgs ca = chunk_args<float>(pl, c, 0, nfc);
-        qf.add_units(i, fft_units(c));
+        qf.add_units(i, fft_units(pl, c));
         qtr.add_units(i, coef_transpose_units<float2>(so3_ncoef(c->B), nfc));
         // split-BF16 plans run the split kernels (kind 4) at every bandlimit
         const bool split = pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6;

@@ -70,7 +70,10 @@ __device__ __forceinline__ void r16_pass(float2 *buf, const float2 *twl, int tl)
 }
 
 // Forward: slice u (beta index k = u / nfc, function fi = u % nfc) of the class, grid -> the
-// 4-pair records of its B^2 groups in the workspace (ws_rec, two float4 per record).
+// 4-pair records of its B^2 groups in the workspace (ws_rec, two float4 per record).  Real
+// functions go in pairs (k = u / nfp, functions fi = 2 (u % nfp) and fi + 1): z = a + i b is one
+// complex slice, and since X(-p) = conj X(p) for a real slice, X_a(p) = (Z(p) + conj Z(-p)) / 2 and
+// X_b(p) = -i (Z(p) - conj Z(-p)) / 2; a record holds the bins p and -p of both its couples.
 template <int R, bool REAL>
 __device__ __forceinline__ void r16_fwd_slice(const FftArgs<float> &a, int u, bool active, float2 *buf,
                                               const float2 *twl, int tl)
@@ -78,15 +81,20 @@ __device__ __forceinline__ void r16_fwd_slice(const FftArgs<float> &a, int u, bo
     constexpr int N = R16<R>::N, P = R16<R>::P, T = R16<R>::T, NN = N * N, B = N / 2;
     constexpr int PER = NN / T;                                  // 16 elements per thread
     const ChunkArgs &ca = a.ca;
-    const int k = u / ca.nfc, fi = u - k * ca.nfc;
+    const int nf = REAL ? (ca.nfc + 1) / 2 : ca.nfc;
+    const int k = u / nf, fi = REAL ? 2 * (u - k * nf) : u - k * nf;
+    const bool two = REAL && fi + 1 < ca.nfc;
     if (active) {
         const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
+        const long base2 = two ? ca.grid_off[ca.fn[fi + 1]] + (long)k * NN : base;
         float2 v[PER];
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int e = tl + i * T;
-            if (REAL) v[i] = make_float2(reinterpret_cast<const float *>(a.src)[base + e], 0.f);
-            else v[i] = reinterpret_cast<const float2 *>(a.src)[base + e];
+            if (REAL) {
+                const float *g = reinterpret_cast<const float *>(a.src);
+                v[i] = make_float2(g[base + e], two ? g[base2 + e] : 0.f);
+            } else v[i] = reinterpret_cast<const float2 *>(a.src)[base + e];
         }
 #pragma unroll
         for (int i = 0; i < PER; i++) {
@@ -107,13 +115,34 @@ __device__ __forceinline__ void r16_fwd_slice(const FftArgs<float> &a, int u, bo
             const float2 p0 = buf[um * P + un], p1 = buf[vm * P + vn];
             const float2 p2 = buf[vn * P + vm], p3 = buf[un * P + um];
             const long o = ws_rec(ca, g, k, fi) * 2;
-            S4[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
-            S4[o + 1] = make_float4(p2.x, p2.y, p3.x, p3.y);
+            if (REAL) {
+                auto sep = [](float2 z, float2 w, float2 &xa, float2 &xb) {   // w = Z(-p)
+                    xa = make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y));
+                    xb = make_float2(0.5f * (z.y + w.y), 0.5f * (w.x - z.x));
+                };
+                float2 a0, b0, a1, b1, a2, b2, a3, b3;
+                sep(p0, p1, a0, b0);
+                sep(p1, p0, a1, b1);
+                sep(p2, p3, a2, b2);
+                sep(p3, p2, a3, b3);
+                S4[o] = make_float4(a0.x, a0.y, a1.x, a1.y);
+                S4[o + 1] = make_float4(a2.x, a2.y, a3.x, a3.y);
+                if (two) {
+                    const long o2 = ws_rec(ca, g, k, fi + 1) * 2;
+                    S4[o2] = make_float4(b0.x, b0.y, b1.x, b1.y);
+                    S4[o2 + 1] = make_float4(b2.x, b2.y, b3.x, b3.y);
+                }
+            } else {
+                S4[o] = make_float4(p0.x, p0.y, p1.x, p1.y);
+                S4[o + 1] = make_float4(p2.x, p2.y, p3.x, p3.y);
+            }
         }
     }
 }
 
 // Inverse: gather the records of slice u into bins (m mod N, n mod N), 2-D DFT, store the slice.
+// Real functions go in pairs as in the forward: the bins of X_a + i X_b transform to the grid of
+// function fi in the real part and that of fi + 1 in the imaginary part.
 template <int R, bool REAL>
 __device__ __forceinline__ void r16_inv_slice(const FftArgs<float> &a, int u, bool active, float2 *buf,
                                               const float2 *twl, int tl)
@@ -121,7 +150,9 @@ __device__ __forceinline__ void r16_inv_slice(const FftArgs<float> &a, int u, bo
     constexpr int N = R16<R>::N, P = R16<R>::P, T = R16<R>::T, NN = N * N, B = N / 2;
     constexpr int PER = NN / T;
     const ChunkArgs &ca = a.ca;
-    const int k = u / ca.nfc, fi = u - k * ca.nfc;
+    const int nf = REAL ? (ca.nfc + 1) / 2 : ca.nfc;
+    const int k = u / nf, fi = REAL ? 2 * (u - k * nf) : u - k * nf;
+    const bool two = REAL && fi + 1 < ca.nfc;
     // the Nyquist row and column (m or n = +-B) are zero; every other bin holds one pair
     for (int i = tl; i < N; i += T) {
         buf[B * P + i] = make_float2(0.f, 0.f);
@@ -136,6 +167,12 @@ __device__ __forceinline__ void r16_inv_slice(const FftArgs<float> &a, int u, bo
             const long o = ws_rec(ca, g, k, fi) * 2;
             q0 = G4[o];
             q1 = G4[o + 1];
+            if (two) {   // z = x_a + i x_b
+                const long o2 = ws_rec(ca, g, k, fi + 1) * 2;
+                const float4 r0 = G4[o2], r1 = G4[o2 + 1];
+                q0 = make_float4(q0.x - r0.y, q0.y + r0.x, q0.z - r0.w, q0.w + r0.z);
+                q1 = make_float4(q1.x - r1.y, q1.y + r1.x, q1.z - r1.w, q1.w + r1.z);
+            }
         }
         const int vm = m0, vn = n0 >= 0 ? n0 : N + n0, um = m0 ? N - m0 : 0, un = n0 > 0 ? N - n0 : -n0;
         buf[vm * P + vn] = make_float2(q0.x, q0.y);                    // (m0, n0)
@@ -152,12 +189,15 @@ __device__ __forceinline__ void r16_inv_slice(const FftArgs<float> &a, int u, bo
     r16_pass<R, false>(buf, twl, tl);
     if (active) {
         const long base = ca.grid_off[ca.fn[fi]] + (long)k * NN;
+        const long base2 = two ? ca.grid_off[ca.fn[fi + 1]] + (long)k * NN : base;
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int e = tl + i * T;
             const float2 v = buf[(e / N) * P + e % N];
-            if (REAL) reinterpret_cast<float *>(a.dst)[base + e] = v.x;
-            else reinterpret_cast<float2 *>(a.dst)[base + e] = v;
+            if (REAL) {
+                reinterpret_cast<float *>(a.dst)[base + e] = v.x;
+                if (two) reinterpret_cast<float *>(a.dst)[base2 + e] = v.y;
+            } else reinterpret_cast<float2 *>(a.dst)[base + e] = v;
         }
     }
 }
@@ -174,10 +214,17 @@ __device__ __forceinline__ void r16_role(int tid, int u0, int nslice, float2 *re
     active = sub < R16<R, NTHR>::SUB && u < nslice;
 }
 
-// One class's slices through NTHR-thread R16 workgroups: item i covers slices [i SUB, (i+1) SUB).
-template <int R, int NTHR = 1024> __host__ __device__ inline int r16_units(int nfc)
+// Slices of a class: N per function, or per pair of functions for real functions (two real slices
+// run as one complex DFT).
+template <int R> __host__ __device__ inline int r16_slices(int nfc, bool real)
 {
-    return (nfc * R16<R>::N + R16<R, NTHR>::SUB - 1) / R16<R, NTHR>::SUB;
+    return (real ? (nfc + 1) / 2 : nfc) * R16<R>::N;
+}
+
+// One class's slices through NTHR-thread R16 workgroups: item i covers slices [i SUB, (i+1) SUB).
+template <int R, int NTHR = 1024> __host__ __device__ inline int r16_units(int nfc, bool real)
+{
+    return (r16_slices<R>(nfc, real) + R16<R, NTHR>::SUB - 1) / R16<R, NTHR>::SUB;
 }
 
 template <int R, bool FWD, bool REAL, int NTHR = 1024>
@@ -194,7 +241,7 @@ __device__ __forceinline__ void r16_item(const FftArgs<float> &a, int item, unsi
     float2 *buf;
     int tl, u;
     bool active;
-    r16_role<R, NTHR>(tid, item * R16<R, NTHR>::SUB, a.ca.nfc * R16<R>::N, regions, buf, tl, u, active);
+    r16_role<R, NTHR>(tid, item * R16<R, NTHR>::SUB, r16_slices<R>(a.ca.nfc, REAL), regions, buf, tl, u, active);
     if (FWD) r16_fwd_slice<R, REAL>(a, u, active, buf, twl, tl);
     else r16_inv_slice<R, REAL>(a, u, active, buf, twl, tl);
 }
@@ -203,7 +250,7 @@ template <int R, bool FWD, bool REAL>
 __global__ void __launch_bounds__(r16_threads<R>()) so3_fft_r16(FftArgs<float> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int n = r16_units<R, r16_threads<R>()>(a.ca.nfc);
+    const int n = r16_units<R, r16_threads<R>()>(a.ca.nfc, REAL);
     const int item = xcd_unit(n);
     if (item >= n) return;
     r16_item<R, FWD, REAL, r16_threads<R>()>(a, item, smem_raw);
