@@ -966,13 +966,19 @@ int run_mx(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fwd, const voi
     } else {
         {
             StageTimer t(pl, s, 3);
-            if (real) {
-                if (s2) hipLaunchKernelGGL((s2_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
-                else hipLaunchKernelGGL((so3_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
-                HIP_TRY(hipGetLastError());
+            if (real && !s2) {
+                // real SO(3): the real mixing and the layout pass in one (so3_coef_real_inv)
+                hipLaunchKernelGGL(so3_coef_real_inv, dim3(xcd_grid(so3_coef_real_units(c->B, ca.nfc))), dim3(256),
+                                   so3_coef_real_lds(c->B), s, ca, reinterpret_cast<const float *>(in),
+                                   reinterpret_cast<float2 *>(pl->d_fc.p));
+            } else {
+                if (real) {
+                    hipLaunchKernelGGL((s2_tmix<float, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_tmp.p, nc);
+                    HIP_TRY(hipGetLastError());
+                }
+                hipLaunchKernelGGL((coef_transpose<1, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
+                                   pl->d_fc.p);
             }
-            hipLaunchKernelGGL((coef_transpose<1, float2>), dim3(tgrid), dim3(256), 0, s, ct, ncp, real ? pl->d_tmp.p : in,
-                               pl->d_fc.p);
             HIP_TRY(hipGetLastError());
         }
         StageTimer t(pl, s, 4);
@@ -1739,6 +1745,7 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_inv_mxk<2, 8>));
     ALLOW_LDS((contract_inv_mxk64<4>));
     ALLOW_LDS((contract_inv_mxk64<2>));
+    ALLOW_LDS(so3_coef_real_inv);
     ALLOW_LDS((contract_fwd_mxks<4, 2>));
     ALLOW_LDS((contract_fwd_mxks<4, 3>));
     ALLOW_LDS((contract_fwd_mxks<2, 2>));

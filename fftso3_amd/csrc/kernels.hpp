@@ -939,6 +939,62 @@ __global__ void so3_tmix(ChunkArgs ca, const void *src, void *dst, long ws_strid
         so3_tmix_elem<T, DIR>(ca, src, dst, ws_stride, e);
 }
 
+// Inverse prologue of real SO(3) plans straight into the batch-wide coefficient staging
+// PT[e][function] (row stride C): the real mixing fused with the layout pass, so the complex packed
+// coefficients never go to memory.  Unit = (degree l, row pair n = +-b) x a tile of 16 functions
+// (one 128-byte PT line per packed index): Fc(l, m, n) = sum conj T(a, m) Fr(a, c) T(c, n) over
+// a = +-|m|, c = +-|n| (src/fdcl_FFTSO3.cpp:1169-1181, as so3_tmix) takes only the rows n = b and
+// n = -b of the degree block.  Each thread reads (row, m) positions of the pair for the tile's
+// functions into LDS; then 16 lanes per packed index form Fc and write whole PT lines.  (The
+// forward direction keeps coef_transpose + so3_tmix: a fused form wrote each function's real rows
+// as 508-byte runs whose partial end lines other units complete at other times, and was slower.)
+#define SCR_TF 16
+__host__ __device__ inline int so3_coef_real_units(int B, int nfc) { return B * (B + 1) / 2 * ((nfc + SCR_TF - 1) / SCR_TF); }
+__host__ __device__ inline size_t so3_coef_real_lds(int B) { return (size_t)2 * (2 * B - 1) * (SCR_TF + 1) * sizeof(float); }
+
+__global__ void __launch_bounds__(256) so3_coef_real_inv(ChunkArgs ca, const float *in, float2 *PT)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int TF = SCR_TF, TP = TF + 1;
+    const int ntf = (ca.nfc + TF - 1) / TF, nunit = so3_coef_real_units(ca.B, ca.nfc);
+    const int u = xcd_unit(nunit);
+    if (u >= nunit) return;
+    const int lb = u / ntf, f0 = (u - lb * ntf) * TF;
+    int l = (int)((sqrtf(8.f * (float)lb + 1.f) - 1.f) * 0.5f);
+    while (l * (l + 1) / 2 > lb) l--;
+    while ((l + 1) * (l + 2) / 2 <= lb) l++;
+    const int b = lb - l * (l + 1) / 2, W = 2 * l + 1, R = b ? 2 : 1, nb = b ? 2 : 1;
+    const int nf = min(TF, ca.nfc - f0);
+    const long base = so3_off(l);
+    const int t = threadIdx.x;
+    // element (row r: n = b or -b, column mi = m + l) of the pair for function f0 + fl
+    auto tix = [&](int r, int mi, int fl) { return (r * W + mi) * TP + fl; };
+    float *tile = reinterpret_cast<float *>(smem_raw);
+    for (int x = t; x < R * W; x += 256) {
+        const int tr = x >= W ? 1 : 0, tmi = x - tr * W, n = tr ? -b : b;
+        const long epos = base + tmi + (long)(n + l) * W;   // packed index of (l, m, n)
+        for (int fl = 0; fl < nf; fl++) tile[tix(tr, tmi, fl)] = in[ca.coef_off[ca.fn[f0 + fl]] + epos];
+    }
+    __syncthreads();
+    for (int i = t; i < R * W * TF; i += 256) {
+        const int fl = i & (TF - 1), x = i / TF, r = x >= W ? 1 : 0, mi = x - r * W;
+        if (fl >= nf) continue;
+        const int m = mi - l, n = r ? -b : b, a = m < 0 ? -m : m, na = a ? 2 : 1;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int ia = 0; ia < na; ia++) {
+            const int xa = ia ? -a : a;
+            const float2 tm = tmat<float2>(xa, m);
+            for (int ib = 0; ib < nb; ib++) {
+                const float2 tn = tmat<float2>(ib ? -b : b, n);
+                const float fv = tile[tix(ib, xa + l, fl)];
+                acc = cadd(acc, cmul(make_float2(tm.x * fv, -tm.y * fv), tn));
+            }
+        }
+        const long e = base + mi + (long)(n + l) * W;
+        PT[e * ca.C + f0 + fl] = acc;
+    }
+}
+
 // S^2: forward Fr(l,a) = Re sum_{c=+-a} conj T(a,c) Fc(l,c) (src/fdcl_FFTS2.cpp:424-436);
 // inverse Fc(l,a) = sum_{c=+-a} T(c,a) Fr(l,c).
 template <typename T, int DIR>
