@@ -2116,9 +2116,10 @@ int fftso3_plan_set_graph(fftso3_plan_t plan, int on)
     return FFTSO3_OK;
 }
 
-// Times inverse + forward (reps each after one warm-up) under each candidate schedule and keeps
+// Times inverse + forward (reps each after two warm-ups) under each candidate schedule and keeps
 // the fastest: for batch-wide classes the workspace row blocking and the group split; for the
-// chunked kernels the chunk size.  The two arrays serve as scratch.
+// chunked kernels the chunk size.  The two arrays serve as scratch.  With graph mode on the pairs
+// are graph replays, as the plan's later calls will be.
 int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps)
 {
     if (int st = check_valid(plan)) return st;
@@ -2160,9 +2161,11 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         cands.push_back({true, 8, 0, 1, 0});
     }
     if (cands.empty()) return FFTSO3_OK;
+    // candidates run as the plan will: with graph mode on, the warm-up pair captures each
+    // candidate's graphs and the timed pairs replay them, so schedules with many small launches are
+    // not charged launch costs that replays do not pay
     const bool timing = plan->timing, graph = plan->graph;
     plan->timing = false;
-    plan->graph = false;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -2178,7 +2181,11 @@ int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *s
         plan->multi_on = c.multi;
         plan->uni_pc = c.pc;
         if ((st = finalize_plan(plan))) break;
-        if ((st = run_dispatch(plan, false, d_coef, d_grid, s)) || (st = run_dispatch(plan, true, d_grid, d_coef, s))) break;
+        // two warm-up pairs: with graph mode the first captures the graphs and the second is their
+        // first replay, which also uploads them (a cost that grows with the launch count)
+        for (int w = 0; w < 2 && st == FFTSO3_OK; w++)
+            if ((st = run_dispatch(plan, false, d_coef, d_grid, s)) == FFTSO3_OK) st = run_dispatch(plan, true, d_grid, d_coef, s);
+        if (st) break;
         (void)hipEventRecord(e0, s);
         for (int r = 0; r < reps && st == FFTSO3_OK; r++) {
             st = run_dispatch(plan, false, d_coef, d_grid, s);

@@ -105,8 +105,10 @@ int fftso3_plan_set_graph(fftso3_plan_t plan, int on);
 
 /* Schedule tuning: times `reps` inverse + forward pairs under each candidate schedule (chunk size
  * of the chunked kernels; row blocking of the spectrum workspace and group split of the
- * batch-wide kernels) and keeps the fastest.  d_grid and d_coef serve as scratch: their contents
- * are undefined afterwards.  No counterpart in the reference. */
+ * batch-wide kernels; for several bandlimits, one launch per stage or per class) and keeps the
+ * fastest.  With graph mode on (and a non-NULL stream) the timed pairs are graph replays.  d_grid
+ * and d_coef serve as scratch: their contents are undefined afterwards.  No counterpart in the
+ * reference. */
 int fftso3_plan_autotune(fftso3_plan_t plan, void *d_grid, void *d_coef, void *stream, int reps);
 
 /* Schedule of the batch-wide kernels, the space autotune searches: workspace row blocking klo
