@@ -951,7 +951,7 @@ __host__ __device__ inline size_t mxk_lds_bytes(int N, int B, int NP, bool fwd)
     const int N32 = (N + 31) & ~31;
     const size_t table = (size_t)NP * MXK_MAXROWS * 4;
     if (fwd) return (size_t)32 * ((B + 31) / 32) * (N32 + 1) * 4 + table;
-    return (size_t)((B + 31) & ~31) * (N + 1) * 4 + table;
+    return (size_t)((B + 15) & ~15) * (N + 1) * 4 + table;
 }
 
 template <int NP, int NW>
@@ -1056,9 +1056,9 @@ __global__ void __launch_bounds__(512) contract_fwd_mxk(MxArgs a)
 }
 
 
-// Inverse with the group's c_l d^l block resident in LDS ([M32][N+1], rows l - m0 zero-padded to a
-// multiple of 32, so each group of 16 MFMA steps runs whole); wave w produces block columns
-// [32w, 32w+32) for every beta row.
+// Inverse with the group's c_l d^l block resident in LDS ([M16][N+1], rows l - m0 zero-padded to a
+// multiple of 16, so each half of a group of 16 MFMA steps runs whole); wave w produces block
+// columns [32w, 32w+32) for every beta row.
 // NW waves per workgroup, each owning 32 columns of a block: 8 waves (2 per SIMD) share the
 // group's d block so one wave's MFMAs cover another's LDS and HBM waits.
 template <int NP, int NW, bool PF = true>
@@ -1072,9 +1072,9 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
     if (!mx_unit(a, nfb, unit, g, fb0, fb1)) return;
     int m0, n0;
     mx_group<NP>(g, m0, n0);
-    const int M = B - m0, M32 = (M + 31) & ~31, DP = N + 1, nkt = (N + 31) / 32;
-    float *D = reinterpret_cast<float *>(smem_raw);                 // [M32][DP], rows >= M zero
-    int *etab = reinterpret_cast<int *>(D + (size_t)M32 * DP);      // [NP][MXK_MAXROWS]
+    const int M = B - m0, M16 = (M + 15) & ~15, DP = N + 1, nkt = (N + 31) / 32;
+    float *D = reinterpret_cast<float *>(smem_raw);                 // [M16][DP], rows >= M zero
+    int *etab = reinterpret_cast<int *>(D + (size_t)M16 * DP);      // [NP][MXK_MAXROWS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
     for (int i = tid; i < NP * MXK_MAXROWS; i += 64 * NW) {
@@ -1089,9 +1089,9 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
             group_recursion_lds(a.rt, rc, NP == 2, g, m0, n0, k, B, [&](int l, double v) {
                 D[(l - m0) * DP + k] = (float)(NP == 2 ? v : (double)(2 * l + 1) * v);
             });
-            for (int r = M; r < M32; r++) D[r * DP + k] = 0.f;
+            for (int r = M; r < M16; r++) D[r * DP + k] = 0.f;
         } else if (k < DP) {
-            for (int r = 0; r < M32; r++) D[r * DP + k] = 0.f;
+            for (int r = 0; r < M16; r++) D[r * DP + k] = 0.f;
         }
     }
     __syncthreads();
@@ -1128,16 +1128,24 @@ __device__ __forceinline__ void contract_inv_mxk_body(const MxArgs &a, int unit)
             // the next row (or the index table): finite or not, they only reach rows k >= N of
             // the result, which are never stored
             const float *Dt = D + kh * DP + 32 * t + cl;
-            // 16 operand reads from LDS ahead of their 16 MFMAs, so no MFMA waits on its own read
+            // 16 operand reads from LDS ahead of their 16 MFMAs, so no MFMA waits on its own read;
+            // the second 8 steps (16 degrees) of a group only where the group's degrees reach them
 #pragma unroll
             for (int s0 = 0; s0 < 64; s0 += 16) {
                 if (2 * s0 < M) {
+                    const bool hi = 2 * (s0 + 8) < M;
                     float d[16];
 #pragma unroll
-                    for (int u = 0; u < 16; u++) d[u] = Dt[2 * (s0 + u) * DP];
+                    for (int u = 0; u < 8; u++) d[u] = Dt[2 * (s0 + u) * DP];
+                    if (hi)
+#pragma unroll
+                        for (int u = 8; u < 16; u++) d[u] = Dt[2 * (s0 + u) * DP];
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int u = 0; u < 16; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], buf[s0 + u], acc, 0, 0, 0);
+                    for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], buf[s0 + u], acc, 0, 0, 0);
+                    if (hi)
+#pragma unroll
+                        for (int u = 8; u < 16; u++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u], buf[s0 + u], acc, 0, 0, 0);
                 }
             }
             const bool ok = c < NCOL;
