@@ -839,6 +839,146 @@ __device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int uni
     }
 }
 
+// contract_inv_mx32g with blocks of 16 functions: each wave computes two 32-column tiles (functions
+// 0-7 and 8-15 of the block) of its group and pair couple, so a lane's two operand loads of one
+// degree read the two 64-byte halves of a 128-byte coefficient line back to back (with blocks of 8
+// functions the other half was read a block later).  The 256-column output block is staged in
+// LDS 64 beta rows at a time (the same 64 KB as the 128 x 128 block of the 8-function form):
+// column c of staged row r at c ^ 8 ((c >> 6) & 1) ^ 4 ((r >> 2) & 1), which keeps the
+// accumulator writes free of bank conflicts and float4 groups whole, and each row goes out as one
+// contiguous 1 KB segment.  The unified multi-class kernel keeps the 8-function form (fewer
+// registers).
+__device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int unit)
+{
+    constexpr int GPW = 2, NP = 4, FB = 16, WPG = 2, OP = 256;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    // D: [GPW][64][MXG_DPITCH], rows l - m0, column k stored at k ^ 32 (l - m0 odd)
+    float *D = reinterpret_cast<float *>(smem_raw);
+    float *O = D + GPW * 64 * MXG_DPITCH;                            // [64][OP]
+    RecCoef *rcs = reinterpret_cast<RecCoef *>(O + 64 * OP);          // [GPW][B]
+    const ChunkArgs &ca = a.ca;
+    const int B = ca.B, C = ca.C, CW = C * NP * 2;
+    const int nfb = (ca.nfc + FB - 1) / FB;
+    int gb, fb0, fb1;
+    if (!mx_unit(a, nfb, unit, gb, fb0, fb1)) return;         // gb: first (even) group of the set
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kh = lane >> 5, cl = lane & 31;
+    const int gw = wave / WPG, pw = wave % WPG;              // group and pair couple of this wave
+    int m0, n0;
+    so3_group(gb + gw, m0, n0);
+    const int M = B - m0, M4 = (M + 3) & ~3;
+    // operand role of this lane: pair p, function fo of each tile, re/im part cl & 1
+    const int p = pw * GPW + (cl >> 4), fo = (cl & 15) >> 1;
+    const float *PT = reinterpret_cast<const float *>(a.src);
+    const MxPair<NP> pr(m0, n0, p);
+    int eoff[32];
+#pragma unroll
+    for (int s = 0; s < 32; s++) eoff[s] = pr.packed(m0 + min(2 * s + kh, M - 1));
+    float bA[2][32], bB[2][32];
+    auto load_block = [&](float (&buf)[2][32], int fb) {
+        const int f0 = fb * FB + fo, f1 = f0 + 8;
+        const float *s0 = PT + 2 * (f0 < ca.nfc ? f0 : 0) + (cl & 1);
+        const float *s1 = PT + 2 * (f1 < ca.nfc ? f1 : 0) + (cl & 1);
+#pragma unroll
+        for (int s = 0; s < 32; s++) {
+            buf[0][s] = s0[2L * eoff[s] * C];
+            buf[1][s] = s1[2L * eoff[s] * C];
+        }
+    };
+    load_block(bA, fb0);
+    {
+        // threads 0-127 recurse for the first group, 128-255 for the second: one beta row each
+        const int h = tid >> 7, k = tid & 127;
+        int hm0, hn0;
+        so3_group(gb + h, hm0, hn0);
+        RecCoef *rc = rcs + h * B;
+        float *Dh = D + h * 64 * MXG_DPITCH;
+        group_coefs(a.rt, 0, B, hm0, hn0, rc, k, 128);
+        __syncthreads();
+        const int hM = B - hm0, hM4 = (hM + 3) & ~3;
+        group_recursion_lds(a.rt, rc, 0, gb + h, hm0, hn0, k, B, [&](int l, double v) {
+            Dh[(l - hm0) * MXG_DPITCH + (k ^ (((l - hm0) & 1) << 5))] = (float)((double)(2 * l + 1) * v);
+        });
+        for (int r = hM; r < hM4; r++) Dh[r * MXG_DPITCH + (k ^ ((r & 1) << 5))] = 0.f;
+    }
+    const float sg = (float)so3_pair_sign(m0, n0, p);
+    // block column of tile c, lane cl: function 8c + fo, group gw, pair p, re/im
+    const int colb = 16 * fo + 8 * gw + 2 * p + (cl & 1);
+    float *G = reinterpret_cast<float *>(a.dst);
+    // the lane's rows 2s + kh have parity kh: column 32t + cl sits at 32 (t ^ kh) + cl
+    const float *Dg = D + gw * 64 * MXG_DPITCH + kh * MXG_DPITCH + cl;
+    const float *DgE = Dg + 32 * kh, *DgO = Dg - 32 * kh;   // even / odd t
+    auto multiply_store = [&](const float (&buf)[2][32], int fb) {
+        f32x16m_t acc[4][2];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) acc[t][c][j] = 0.f;
+        // degrees l >= M read finite coefficients (row M-1) against zero rows of D up to M4
+#pragma unroll
+        for (int s = 0; s < 32; s += 2) {
+            if (4 * (s >> 1) < M4) {
+                float d[2][4];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) d[u][t] = (t & 1 ? DgO : DgE)[(2 * s + 2 * u) * MXG_DPITCH + 32 * t];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+#pragma unroll
+                        for (int c = 0; c < 2; c++)
+                            acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u][t], buf[c][s + u], acc[t][c], 0, 0, 0);
+            }
+        }
+        const bool okb = fb * FB < ca.nfc;
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++) {
+            __syncthreads();                             // previous half's rows stored
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+#pragma unroll
+                    for (int j = 0; j < 16; j++) {
+                        const int r = tt * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);   // row of the half
+                        const int col = 128 * c + colb;
+                        O[r * OP + (col ^ (8 * ((col >> 6) & 1)) ^ (4 * kh))] = sg * acc[2 * hh + tt][c][j];
+                    }
+            __syncthreads();
+            // rows w + 4i of the half, lanes 16-byte groups of one 1 KB row segment
+            int rr = wave;
+            asm volatile("" : "+v"(rr));   // row offsets formed at the stores
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int r = rr + 4 * i, pos = 4 * lane;
+                const int col = pos ^ (8 * ((pos >> 6) & 1)) ^ (4 * ((r >> 2) & 1));   // logical column of pos
+                const float4 v = *reinterpret_cast<const float4 *>(O + r * OP + pos);
+                const int k = 64 * hh + r;
+                const bool ok = okb && fb * FB + (col >> 4) < ca.nfc;
+                float *dst = ok ? G + ws_row(ca, gb, k) * CW + (long)fb * OP + col : a.dummy;
+                *reinterpret_cast<float4 *>(dst) = v;
+            }
+        }
+    };
+    __syncthreads();                                     // D written
+    for (int fb = fb0; fb < fb1; fb += 2) {
+        if (fb + 1 < fb1) load_block(bB, fb + 1);
+        multiply_store(bA, fb);
+        if (fb + 1 >= fb1) break;
+        if (fb + 2 < fb1) load_block(bA, fb + 2);
+        multiply_store(bB, fb + 1);
+    }
+}
+
+__global__ void __launch_bounds__(256) contract_inv_mx32g16(MxArgs a)
+{
+    contract_inv_mx32g16_body(a, mx_block_unit(a));
+}
+
 template <int GPW>
 __global__ void __launch_bounds__(256) contract_inv_mx32g(MxArgs a)
 {
