@@ -7,9 +7,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfftso3_amd.so")
 FDCL_LIB = os.path.join(HERE, "libfftso3_fdcl.so")
 
-HIP_SOURCES = [os.path.join(HERE, "csrc", "api.hip")]
-HIP_DEPS = [os.path.join(HERE, "csrc", f)
-            for f in ("common.hpp", "fft.hpp", "fft128.hpp", "kernels.hpp", "mfma.hpp", "mfma32.hpp", "mfma_otf.hpp")]
+HIP_SOURCES = [os.path.join(HERE, "csrc", "api.hip")]   # includes every csrc/*.hpp
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
