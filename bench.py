@@ -27,9 +27,19 @@ PARITY_BAR = {"fp32": 1e-6, "bf16x6": 1e-6, "bf16x3": 5e-5}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "so3_b64_batch256_fp32_pmc.json")
 # kernels of the batch-wide FP32 path at l_max = 63 with its default schedule (group interleave 1:
 # paired-function FFTs, contractions over the two groups that share a workspace line), by stage
-# (the forward contraction is two launches, <4> for the groups with more than 32 degrees and <2>)
-STAGE_KERNEL = {"fft_fwd": "so3_fft128_fwd2<false>", "contract_fwd": "contract_fwd_mx16g<4>",
-                "contract_inv": "contract_inv_mx32g<2>", "fft_inv": "so3_fft128_inv2<false>"}
+# (the forward contraction is two launches: eight waves for the groups with more than 32 degrees,
+# <2> for the rest)
+STAGE_KERNELS = {"fft_fwd": ["so3_fft128_fwd2<false>"],
+                 "contract_fwd": ["contract_fwd_mx16g8", "contract_fwd_mx16g<2>"],
+                 "contract_inv": ["contract_inv_mx32g16"], "fft_inv": ["so3_fft128_inv2<false>"]}
+
+
+def stage_traffic(kinfo, stage):
+    """Counter traffic (bytes) of one launch of every kernel of a stage, or None if not profiled."""
+    names = STAGE_KERNELS.get(stage, [])
+    if not names or any(n not in kinfo for n in names):
+        return None
+    return sum(kinfo[n]["traffic_bytes"] for n in names)
 
 
 def log(msg):
@@ -471,11 +481,10 @@ def main():
     traffic = None
     kinfo = {}
     default_sched = args.precision == "fp32" and args.chunk is None and " gil=1 " in plan.describe()
-    kname = STAGE_KERNEL[dom] if default_sched else dom
+    kname = " + ".join(STAGE_KERNELS[dom]) if default_sched else dom
     if default_sched and n_fn == 256 and os.path.exists(PMC_SUMMARY):
         kinfo = json.load(open(PMC_SUMMARY))["kernels"]
-        k = kinfo.get(kname)
-        traffic = k["traffic_bytes"] if k else None
+        traffic = stage_traffic(kinfo, dom)
     roofline = {"bound": "hbm", "kernel": kname, "stage": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": bytes_launch, "avg_launch_ms": avg_ms,
@@ -489,9 +498,9 @@ def main():
         ab = alg_bytes(B, C, 8, st)
         row = {"avg_launch_ms": a_ms, "alg_bytes": ab, "frac": ab / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                "workspace_bytes": design_bytes(B, C, 8, st)}
-        kn = STAGE_KERNEL.get(st)
-        if kn and kn in kinfo and ab:
-            row["traffic_over_alg"] = kinfo[kn]["traffic_bytes"] / ab
+        tr = stage_traffic(kinfo, st)
+        if tr and ab:
+            row["traffic_over_alg"] = tr / ab
         per_kernel[st] = row
     extra["per_kernel_roofline"] = per_kernel
     whole = UNIT_BYTES_B64 * value / 1e9

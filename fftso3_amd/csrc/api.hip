@@ -865,17 +865,18 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
             const int ggrid = deal_grid(ag.nitems * ag.nsplit, ag.nsplit);
             const size_t glds = mx16g_lds_bytes(c->B);
             if (ag.gpw == 2) {
-                // groups with at most 32 degrees (m0 >= B - 32; 3/4 of them at l_max = 63) keep
-                // two 16-row tiles of D in registers instead of four: fewer registers, two
-                // workgroups per CU
+                // every wave keeps two 16-row tiles of D in registers: the groups with more than 32
+                // degrees run on eight waves (two per group and pair couple, one per tile half, at
+                // two waves per SIMD), the groups with at most 32 (m0 >= B - 32; 3/4 of them at
+                // l_max = 63) on four, two workgroups per CU
                 const int gsplit = c->B > 32 ? (c->B - 32) * (c->B - 32) : 0;
                 MxArgs a4 = ag, a2 = ag;
                 a4.nitems = gsplit / 2;
                 a2.g0 = gsplit;
                 a2.nitems = ag.nitems - gsplit / 2;
                 if (a4.nitems > 0)
-                    hipLaunchKernelGGL((contract_fwd_mx16g<4>), dim3(deal_grid(a4.nitems * a4.nsplit, a4.nsplit)),
-                                       dim3(256), glds, s, a4);
+                    hipLaunchKernelGGL(contract_fwd_mx16g8, dim3(deal_grid(a4.nitems * a4.nsplit, a4.nsplit)),
+                                       dim3(512), glds, s, a4);
                 if (a2.nitems > 0)
                     hipLaunchKernelGGL((contract_fwd_mx16g<2>), dim3(deal_grid(a2.nitems * a2.nsplit, a2.nsplit)),
                                        dim3(256), glds, s, a2);
@@ -1729,7 +1730,7 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32g<4>));
     ALLOW_LDS((contract_inv_mx32g<2>));
     ALLOW_LDS(contract_inv_mx32g16);
-    ALLOW_LDS((contract_fwd_mx16g<4>));
+    ALLOW_LDS(contract_fwd_mx16g8);
     ALLOW_LDS((contract_fwd_mx16g<2>));
     ALLOW_LDS((contract_fwd_tab<4>));
     ALLOW_LDS((contract_fwd_tab<2>));

@@ -425,10 +425,12 @@ typedef float f32x4m_t __attribute__((ext_vector_type(4)));
 // (44% of the groups at l_max = 63).  Tile layouts (lane i): A[i%16][i/16] = D(l, k), B[i/16][i%16]
 // = spectrum(k, column), result row 4(i/16) + j, column i%16.  Wave w takes group w / 2 and pairs
 // 2(w % 2), 2(w % 2) + 1: one 16-column tile per pair (8 functions, re/im).
-template <int NRT, bool PAIR = true>
+// NH = 2: eight waves, waves w and w + 4 splitting the group's row tiles (w + 4 takes tiles NRT ..
+// 2 NRT - 1), so a group with up to 2 NRT x 16 degrees runs at two waves per SIMD.
+template <int NRT, bool PAIR = true, int NH = 1>
 __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int unit)
 {
-    constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2;
+    constexpr int GPW = 2, NP = 4, FB = 8, WPG = 2, RPI = 8 * NH;   // RPI: block rows per load step
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float *L = reinterpret_cast<float *>(smem_raw);     // [128][MXG_LPITCH], spectrum block
     float *Dst = L;                                      // [2][64][MX_DPITCH] staging, before L is used
@@ -440,48 +442,54 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
     if (!mx_unit(a, nfb, unit, gb, fb0, fb1)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kq = lane >> 4, cx = lane & 15;
-    const int gg = wave / WPG, pg = wave % WPG;
+    const int wq = wave & 3, th = wave >> 2;             // th: row-tile half (NH = 2)
+    const int gg = wq / WPG, pg = wq % WPG;
     int m0, n0;
     so3_group(gb + gg, m0, n0);
     const int M = B - m0, nrt = (M + 15) / 16;
-    // load role (as contract_fwd_mx32g<2>): rows r0 + 8i of the block, columns 4q..4q+3
+    const int t0 = NRT * th, nrw = min(max(nrt - t0, 0), NRT);   // this wave's row tiles t0 + [0, nrw)
+    // load role (as contract_fwd_mx32g<2>): rows r0 + RPI i of the block, columns 4q..4q+3
     const int q = tid & 31, r0 = tid >> 5;
     const int fl_q = (q >> 1) / GPW, gq = (q >> 1) % GPW;
     auto lcol = [](int fl, int gq_, int p) { return (gq_ * WPG + p / GPW) * 32 + (p % GPW) * FB * 2 + fl * 2; };
     const int lc0 = lcol(fl_q, gq, 2 * (q & 1)), lc1 = lcol(fl_q, gq, 2 * (q & 1) + 1);
     const float *S = reinterpret_cast<const float *>(a.src);
-    // rows r0 + 8i (< N = 128): the workspace rows are [k][g] (no k blocking with the interleave),
-    // so their offsets step by a constant
-    const long rbase = ws_row(ca, gb, r0) * CW + 4 * q, rstep = (ws_row(ca, gb, r0 + 8) - ws_row(ca, gb, r0)) * CW;
-    float4 pf[16];
+    // rows r0 + RPI i (< N = 128): the workspace rows are [k][g] (no k blocking with the
+    // interleave), so their offsets step by a constant
+    const long rbase = ws_row(ca, gb, r0) * CW + 4 * q, rstep = (ws_row(ca, gb, r0 + RPI) - ws_row(ca, gb, r0)) * CW;
+    float4 pf[16 / NH];
     auto load_block = [&](int fb) {
         const long co = fb * FB + fl_q < ca.nfc ? (long)fb * 128 : 0;
 #pragma unroll
-        for (int i = 0; i < 16; i++) pf[i] = *reinterpret_cast<const float4 *>(S + rbase + i * rstep + co);
+        for (int i = 0; i < 16 / NH; i++) pf[i] = *reinterpret_cast<const float4 *>(S + rbase + i * rstep + co);
     };
     load_block(fb0);
-    // D of the two groups: threads 0-127 recurse for the first, 128-255 the second
+    // D of the two groups: threads 0-127 recurse for the first, 128-255 the second (with NH = 2
+    // threads 256-511 only meet the barriers: the recursion has none of its own)
     float dR[NRT][32];
     {
-        const int h = tid >> 7;
+        const int h = min(tid >> 7, 1);
+        const bool rec = tid < 256;
         int sm0, sn0;
         so3_group(gb + h, sm0, sn0);
         RecCoef *rc = rcs + h * B;
         float *Dh = Dst + h * 64 * MX_DPITCH;
-        group_coefs(a.rt, 0, B, sm0, sn0, rc, tid & 127, 128);
+        if (rec) group_coefs(a.rt, 0, B, sm0, sn0, rc, tid & 127, 128);
         __syncthreads();
-        const int k = tid & 127, sM = B - sm0;
-        const double wk = a.rt.w[k];
-        group_recursion_lds(a.rt, rc, 0, gb + h, sm0, sn0, k, B, [&](int l, double v) {
-            Dh[(l - sm0) * MX_DPITCH + k] = (float)(wk * v);
-        });
-        for (int r = sM; r < 16 * ((sM + 15) / 16); r++) Dh[r * MX_DPITCH + k] = 0.f;
+        if (rec) {
+            const int k = tid & 127, sM = B - sm0;
+            const double wk = a.rt.w[k];
+            group_recursion_lds(a.rt, rc, 0, gb + h, sm0, sn0, k, B, [&](int l, double v) {
+                Dh[(l - sm0) * MX_DPITCH + k] = (float)(wk * v);
+            });
+            for (int r = sM; r < 16 * ((sM + 15) / 16); r++) Dh[r * MX_DPITCH + k] = 0.f;
+        }
         __syncthreads();
         const float *D0 = Dst + gg * 64 * MX_DPITCH + cx * MX_DPITCH + kq;
 #pragma unroll
         for (int t = 0; t < NRT; t++)
 #pragma unroll
-            for (int s = 0; s < 32; s++) dR[t][s] = t < nrt ? D0[16 * t * MX_DPITCH + 4 * s] : 0.f;
+            for (int s = 0; s < 32; s++) dR[t][s] = t < nrw ? D0[16 * (t0 + t) * MX_DPITCH + 4 * s] : 0.f;
         __syncthreads();                                 // staging read before it is overwritten
     }
     // output role of this lane: pairs pg*2 + {0, 1} (column tiles 0, 1), function fo, re/im cx & 1
@@ -492,14 +500,14 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
     float *PT = reinterpret_cast<float *>(a.dst);
     // operand rows 4 s + kq: columns swizzled by 16 kq
     f32x4m_t accp[NRT][2];   // results of the first block of a pair
-    const float *Lw0 = L + kq * MXG_LPITCH + ((wave * 32 + cx) ^ (16 * kq));
-    const float *Lw1 = L + kq * MXG_LPITCH + ((wave * 32 + 16 + cx) ^ (16 * kq));
+    const float *Lw0 = L + kq * MXG_LPITCH + ((wq * 32 + cx) ^ (16 * kq));
+    const float *Lw1 = L + kq * MXG_LPITCH + ((wq * 32 + 16 + cx) ^ (16 * kq));
     const int lsw = 16 * (r0 & 3);
     for (int fb = fb0; fb < fb1; fb++) {
         __syncthreads();                                 // previous block consumed
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            float *row = L + (r0 + 8 * i) * MXG_LPITCH;   // (r0 + 8i) & 3 = r0 & 3
+        for (int i = 0; i < 16 / NH; i++) {
+            float *row = L + (r0 + RPI * i) * MXG_LPITCH;   // (r0 + RPI i) & 3 = r0 & 3
             *reinterpret_cast<float2 *>(row + (lc0 ^ lsw)) = make_float2(pf[i].x, pf[i].y);
             *reinterpret_cast<float2 *>(row + (lc1 ^ lsw)) = make_float2(pf[i].z, pf[i].w);
         }
@@ -532,10 +540,10 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
                     }
             }
         };
-        if (nrt == 1) run(std::integral_constant<int, 1>());
-        else if (NRT == 2 || nrt == 2) run(std::integral_constant<int, 2>());
-        else if (nrt == 3) run(std::integral_constant<int, NRT >= 3 ? 3 : 2>());
-        else run(std::integral_constant<int, NRT>());
+        if (nrw == 1) run(std::integral_constant<int, 1>());
+        else if (NRT == 2 || nrw == 2) run(std::integral_constant<int, 2>());
+        else if (nrw == 3) run(std::integral_constant<int, NRT >= 3 ? 3 : 2>());
+        else if (nrw > 0) run(std::integral_constant<int, NRT>());
         // blocks 2i and 2i + 1 fill the two 64-byte halves of each 128-byte line of PT (16
         // functions): the first block's results wait in registers so both halves are stored by
         // back-to-back instructions
@@ -553,10 +561,10 @@ __device__ __forceinline__ void contract_fwd_mx16g_body(const MxArgs &a, int uni
         float *o = PT + 2 * fn + (cx & 1);
 #pragma unroll
         for (int t = 0; t < NRT; t++) {
-            if (t < nrt) {
+            if (t < nrw) {
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const int r = 16 * t + 4 * kq + j;
+                    const int r = 16 * (t0 + t) + 4 * kq + j;
                     const long e0 = pr0.packed(m0 + min(r, M - 1)), e1 = pr1.packed(m0 + min(r, M - 1));
                     const bool ok0 = pv0 && r < M, ok1 = pv1 && r < M;
                     if (paired) {   // the previous block: functions 8 lower, always < nfc
@@ -579,6 +587,12 @@ template <int NRT>
 __global__ void __launch_bounds__(256, NRT == 2 ? 2 : 1) contract_fwd_mx16g(MxArgs a)
 {
     contract_fwd_mx16g_body<NRT>(a, mx_block_unit(a));
+}
+
+// the groups with more than 32 degrees on eight waves, two row tiles each
+__global__ void __launch_bounds__(512) contract_fwd_mx16g8(MxArgs a)
+{
+    contract_fwd_mx16g_body<2, true, 2>(a, mx_block_unit(a));
 }
 
 // Inverse: G_k(fn, pair) = s_p sum_l c_l d^l_{m0 n0}(beta_k) PT(packed(l, pair); fn) (c_l = 2l+1

@@ -98,9 +98,10 @@ __global__ void __launch_bounds__(1024) so3_fft_multi(MultiFft m)
     });
 }
 
-// contraction kinds: forward 0 = contract_fwd_mx32<4, false> (N < 128), 2 = contract_fwd_mx16g<4>,
-// 3 = contract_fwd_mx16g<2> (N = 128, group interleave 1); inverse 0 = contract_inv_mx32<4>,
-// 1 = contract_inv_mx32g<2>; split plans 4 = contract_fwd_mxks / contract_inv_mxks
+// contraction kinds: forward 0 = contract_fwd_mx32<4, false> (N < 128), 2 = the contract_fwd_mx16g
+// body with four row tiles per wave, 3 = with two (N = 128, group interleave 1); inverse 0 =
+// contract_inv_mx32<4>, 1 = contract_inv_mx32g<2>; split plans 4 = contract_fwd_mxks /
+// contract_inv_mxks
 struct MultiMx {
     MxArgs c[MULTI_MAX_MX];
     int kind[MULTI_MAX_MX];
