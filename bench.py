@@ -31,7 +31,7 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "so3_b64_batch256_fp32_pmc.json")
 # <2> for the rest)
 STAGE_KERNELS = {"fft_fwd": ["so3_fft128_fwd2<false>"],
                  "contract_fwd": ["contract_fwd_mx16g8", "contract_fwd_mx16g<2>"],
-                 "contract_inv": ["contract_inv_mx32g16"], "fft_inv": ["so3_fft128_inv2<false>"]}
+                 "contract_inv": ["contract_inv_mx32g16w8"], "fft_inv": ["so3_fft128_inv2<false>"]}
 
 
 def stage_traffic(kinfo, stage):

@@ -924,10 +924,10 @@ int launch_mx_contract(fftso3_plan_s *pl, Class *c, const ChunkArgs &ca, bool fw
             MxArgs ag = a;
             ag.gpw = 2;
             ag.nitems = c->nitems_mx / 2;
-            const int nfbg = (ca.nfc + 15) / 16;   // blocks of 16 functions (contract_inv_mx32g16)
+            const int nfbg = (ca.nfc + 15) / 16;   // blocks of 16 functions (contract_inv_mx32g16w8)
             ag.nsplit = std::max(1, std::min(nfbg, pl->split_mul * ((2048 + ag.nitems - 1) / ag.nitems)));
             const size_t glds = mx32g_inv_lds_bytes(c->B);
-            hipLaunchKernelGGL(contract_inv_mx32g16, dim3(deal_grid(ag.nitems * ag.nsplit, ag.nsplit)), dim3(256),
+            hipLaunchKernelGGL(contract_inv_mx32g16w8, dim3(deal_grid(ag.nitems * ag.nsplit, ag.nsplit)), dim3(512),
                                glds, s, ag);
         } else hipLaunchKernelGGL((contract_inv_mx32<NP>), dim3(mgrid), dim3(256), lds, s, a);
     }
@@ -1729,7 +1729,7 @@ int set_kernel_attrs()
     ALLOW_LDS((contract_fwd_mx32<4, true>));
     ALLOW_LDS((contract_fwd_mx32g<4>));
     ALLOW_LDS((contract_inv_mx32g<2>));
-    ALLOW_LDS(contract_inv_mx32g16);
+    ALLOW_LDS(contract_inv_mx32g16w8);
     ALLOW_LDS(contract_fwd_mx16g8);
     ALLOW_LDS((contract_fwd_mx16g<2>));
     ALLOW_LDS((contract_fwd_tab<4>));

@@ -853,18 +853,22 @@ __device__ __forceinline__ void contract_inv_mx32g_body(const MxArgs &a, int uni
     }
 }
 
-// contract_inv_mx32g with blocks of 16 functions: each wave computes two 32-column tiles (functions
-// 0-7 and 8-15 of the block) of its group and pair couple, so a lane's two operand loads of one
-// degree read the two 64-byte halves of a 128-byte coefficient line back to back (with blocks of 8
-// functions the other half was read a block later).  The 256-column output block is staged in
-// LDS 64 beta rows at a time (the same 64 KB as the 128 x 128 block of the 8-function form):
-// column c of staged row r at c ^ 8 ((c >> 6) & 1) ^ 4 ((r >> 2) & 1), which keeps the
-// accumulator writes free of bank conflicts and float4 groups whole, and each row goes out as one
-// contiguous 1 KB segment.  The unified multi-class kernel keeps the 8-function form (fewer
-// registers).
+// contract_inv_mx32g with blocks of 16 functions: a block is two 32-column tiles (functions 0-7
+// and 8-15) per group and pair couple.  With NW = 4 waves one wave computes both tiles, so a lane's
+// two operand loads of one degree read the two 64-byte halves of a 128-byte coefficient line back
+// to back (with blocks of 8 functions the other half was read a block later).  With NW = 8 (the
+// form launched) waves w and w + 4 take one tile each: the two halves of a line are loaded by two
+// waves of one SIMD a few cycles apart, and a wave holds half the accumulators and operands (227
+// registers, two waves per SIMD, where NW = 4 needs 467 and runs one).  The 256-column output
+// block is staged in LDS 64 beta rows at a time (the same 64 KB as the 128 x 128 block of the
+// 8-function form): column c of staged row r at c ^ 8 ((c >> 6) & 1) ^ 4 ((r >> 2) & 1), which
+// keeps the accumulator writes free of bank conflicts and float4 groups whole, and each row goes
+// out as one contiguous 1 KB segment.  The unified multi-class kernel keeps the 8-function form
+// (fewer registers).
+template <int NW>
 __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int unit)
 {
-    constexpr int GPW = 2, NP = 4, FB = 16, WPG = 2, OP = 256;
+    constexpr int GPW = 2, NP = 4, FB = 16, WPG = 2, OP = 256, NC = NW == 8 ? 1 : 2;   // NC: tiles per wave
     extern __shared__ __align__(16) unsigned char smem_raw[];
     // D: [GPW][64][MXG_DPITCH], rows l - m0, column k stored at k ^ 32 (l - m0 odd)
     float *D = reinterpret_cast<float *>(smem_raw);
@@ -877,7 +881,8 @@ __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int u
     if (!mx_unit(a, nfb, unit, gb, fb0, fb1)) return;         // gb: first (even) group of the set
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int kh = lane >> 5, cl = lane & 31;
-    const int gw = wave / WPG, pw = wave % WPG;              // group and pair couple of this wave
+    const int wq = wave & 3, ch = NW == 8 ? wave >> 2 : 0;   // ch: the wave's first tile
+    const int gw = wq / WPG, pw = wq % WPG;                  // group and pair couple of this wave
     int m0, n0;
     so3_group(gb + gw, m0, n0);
     const int M = B - m0, M4 = (M + 3) & ~3;
@@ -888,32 +893,36 @@ __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int u
     int eoff[32];
 #pragma unroll
     for (int s = 0; s < 32; s++) eoff[s] = pr.packed(m0 + min(2 * s + kh, M - 1));
-    float bA[2][32], bB[2][32];
-    auto load_block = [&](float (&buf)[2][32], int fb) {
-        const int f0 = fb * FB + fo, f1 = f0 + 8;
+    float bA[NC][32], bB[NC][32];
+    auto load_block = [&](float (&buf)[NC][32], int fb) {
+        const int f0 = fb * FB + 8 * ch + fo, f1 = f0 + 8;
         const float *s0 = PT + 2 * (f0 < ca.nfc ? f0 : 0) + (cl & 1);
         const float *s1 = PT + 2 * (f1 < ca.nfc ? f1 : 0) + (cl & 1);
 #pragma unroll
         for (int s = 0; s < 32; s++) {
             buf[0][s] = s0[2L * eoff[s] * C];
-            buf[1][s] = s1[2L * eoff[s] * C];
+            if (NC == 2) buf[NC - 1][s] = s1[2L * eoff[s] * C];
         }
     };
     load_block(bA, fb0);
     {
         // threads 0-127 recurse for the first group, 128-255 for the second: one beta row each
-        const int h = tid >> 7, k = tid & 127;
+        // (with NW = 8 threads 256-511 only meet the barrier: the recursion has none of its own)
+        const int h = min(tid >> 7, 1), k = tid & 127;
+        const bool rec = tid < 256;
         int hm0, hn0;
         so3_group(gb + h, hm0, hn0);
         RecCoef *rc = rcs + h * B;
         float *Dh = D + h * 64 * MXG_DPITCH;
-        group_coefs(a.rt, 0, B, hm0, hn0, rc, k, 128);
+        if (rec) group_coefs(a.rt, 0, B, hm0, hn0, rc, k, 128);
         __syncthreads();
-        const int hM = B - hm0, hM4 = (hM + 3) & ~3;
-        group_recursion_lds(a.rt, rc, 0, gb + h, hm0, hn0, k, B, [&](int l, double v) {
-            Dh[(l - hm0) * MXG_DPITCH + (k ^ (((l - hm0) & 1) << 5))] = (float)((double)(2 * l + 1) * v);
-        });
-        for (int r = hM; r < hM4; r++) Dh[r * MXG_DPITCH + (k ^ ((r & 1) << 5))] = 0.f;
+        if (rec) {
+            const int hM = B - hm0, hM4 = (hM + 3) & ~3;
+            group_recursion_lds(a.rt, rc, 0, gb + h, hm0, hn0, k, B, [&](int l, double v) {
+                Dh[(l - hm0) * MXG_DPITCH + (k ^ (((l - hm0) & 1) << 5))] = (float)((double)(2 * l + 1) * v);
+            });
+            for (int r = hM; r < hM4; r++) Dh[r * MXG_DPITCH + (k ^ ((r & 1) << 5))] = 0.f;
+        }
     }
     const float sg = (float)so3_pair_sign(m0, n0, p);
     // block column of tile c, lane cl: function 8c + fo, group gw, pair p, re/im
@@ -922,12 +931,12 @@ __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int u
     // the lane's rows 2s + kh have parity kh: column 32t + cl sits at 32 (t ^ kh) + cl
     const float *Dg = D + gw * 64 * MXG_DPITCH + kh * MXG_DPITCH + cl;
     const float *DgE = Dg + 32 * kh, *DgO = Dg - 32 * kh;   // even / odd t
-    auto multiply_store = [&](const float (&buf)[2][32], int fb) {
-        f32x16m_t acc[4][2];
+    auto multiply_store = [&](const float (&buf)[NC][32], int fb) {
+        f32x16m_t acc[4][NC];
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
-            for (int c = 0; c < 2; c++)
+            for (int c = 0; c < NC; c++)
 #pragma unroll
                 for (int j = 0; j < 16; j++) acc[t][c][j] = 0.f;
         // degrees l >= M read finite coefficients (row M-1) against zero rows of D up to M4
@@ -944,7 +953,7 @@ __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int u
 #pragma unroll
                     for (int t = 0; t < 4; t++)
 #pragma unroll
-                        for (int c = 0; c < 2; c++)
+                        for (int c = 0; c < NC; c++)
                             acc[t][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(d[u][t], buf[c][s + u], acc[t][c], 0, 0, 0);
             }
         }
@@ -955,20 +964,20 @@ __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int u
 #pragma unroll
             for (int tt = 0; tt < 2; tt++)
 #pragma unroll
-                for (int c = 0; c < 2; c++)
+                for (int c = 0; c < NC; c++)
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
                         const int r = tt * 32 + 8 * (j >> 2) + 4 * kh + (j & 3);   // row of the half
-                        const int col = 128 * c + colb;
+                        const int col = 128 * (ch + c) + colb;
                         O[r * OP + (col ^ (8 * ((col >> 6) & 1)) ^ (4 * kh))] = sg * acc[2 * hh + tt][c][j];
                     }
             __syncthreads();
-            // rows w + 4i of the half, lanes 16-byte groups of one 1 KB row segment
+            // rows w + NW i of the half, lanes 16-byte groups of one 1 KB row segment
             int rr = wave;
             asm volatile("" : "+v"(rr));   // row offsets formed at the stores
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int r = rr + 4 * i, pos = 4 * lane;
+            for (int i = 0; i < 64 / NW; i++) {
+                const int r = rr + NW * i, pos = 4 * lane;
                 const int col = pos ^ (8 * ((pos >> 6) & 1)) ^ (4 * ((r >> 2) & 1));   // logical column of pos
                 const float4 v = *reinterpret_cast<const float4 *>(O + r * OP + pos);
                 const int k = 64 * hh + r;
@@ -988,9 +997,9 @@ __device__ __forceinline__ void contract_inv_mx32g16_body(const MxArgs &a, int u
     }
 }
 
-__global__ void __launch_bounds__(256) contract_inv_mx32g16(MxArgs a)
+__global__ void __launch_bounds__(512) contract_inv_mx32g16w8(MxArgs a)
 {
-    contract_inv_mx32g16_body(a, mx_block_unit(a));
+    contract_inv_mx32g16_body<8>(a, mx_block_unit(a));
 }
 
 template <int GPW>
