@@ -144,7 +144,7 @@ struct Class {
     bool mxks = false;              // ... split-BF16 K-chunked kernels (128 < N, B <= 128)
     DevBuf d_items_mxs_f, d_items_mxs_i;   // their items: (group, degree range), (group, beta range)
     int nitems_mxs_f = 0, nitems_mxs_i = 0;
-    bool pt = false;                // chunked contraction reads / writes transposed coefficients PT
+    bool pt = false;                // the contraction reads / writes transposed coefficients PT
     DevBuf d_dtab, d_dtab_off;      // precomputed d table of the chunked path at large B (or empty)
     int chunk = 8;                  // functions per pipeline pass for this class
     DevBuf d_items_mx;              // groups of the batch-wide path (largest first)
@@ -389,10 +389,13 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     // MFMA products (mfma_otf.hpp), over half-record workspace rows and PT coefficients
     c->otf = (pl->prec == FFTSO3_BF16X3 || pl->prec == FFTSO3_BF16X6) && pl->domain == FFTSO3_DOMAIN_SO3 &&
              pl->kind == FFTSO3_COMPLEX && N == OTF_N && c->chunk == 8 && !c->mx && !pl->force_chunked;
-    // large bandlimits on the chunked kernels: coefficients go through PT[e][fi] so the contraction
-    // moves whole rows of the chunk's functions instead of scattered single values
+    // large bandlimits on the chunked kernels, and the batch-wide FP64 contractions: coefficients
+    // go through PT[e][fi] so the contraction moves whole rows of the chunk's functions instead of
+    // scattered single values (batch-wide FP64 at l_max = 63 x 256: the inverse read 7.4x its
+    // coefficient bytes as 16-byte pieces of the packed arrays and the forward wrote 1.8x; with PT
+    // and the two layout passes 10,530 -> 11,390 transforms/s)
     c->pt = (!c->mx && pl->kind == FFTSO3_COMPLEX && B > 64 && pl->prec != FFTSO3_BF16X3 &&
-             pl->prec != FFTSO3_BF16X6) || c->otf;
+             pl->prec != FFTSO3_BF16X6) || c->otf || (c->mx64 && pl->kind == FFTSO3_COMPLEX);
     // Above l_max = 127 the chunked contraction would spend its time in the sequential recursion
     // (one lane per k, the rest of the workgroup idle): the plan computes d once, at full
     // occupancy, into a table it keeps ([g][l - m0][k] in the storage type; 11.5 GB at l_max = 255
