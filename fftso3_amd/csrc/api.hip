@@ -393,9 +393,11 @@ int build_class_work(fftso3_plan_s *pl, Class *c)
     // go through PT[e][fi] so the contraction moves whole rows of the chunk's functions instead of
     // scattered single values (batch-wide FP64 at l_max = 63 x 256: the inverse read 7.4x its
     // coefficient bytes as 16-byte pieces of the packed arrays and the forward wrote 1.8x; with PT
-    // and the two layout passes 10,530 -> 11,390 transforms/s)
+    // and the two layout passes 10,530 -> 11,390 transforms/s).  Real FP64 batches stage the
+    // packed complex coefficients of the real mixing in d_tmp, between the mixing and the layout
+    // pass.
     c->pt = (!c->mx && pl->kind == FFTSO3_COMPLEX && B > 64 && pl->prec != FFTSO3_BF16X3 &&
-             pl->prec != FFTSO3_BF16X6) || c->otf || (c->mx64 && pl->kind == FFTSO3_COMPLEX);
+             pl->prec != FFTSO3_BF16X6) || c->otf || c->mx64;
     // Above l_max = 127 the chunked contraction would spend its time in the sequential recursion
     // (one lane per k, the rest of the workgroup idle): the plan computes d once, at full
     // occupancy, into a table it keeps ([g][l - m0][k] in the storage type; 11.5 GB at l_max = 255
@@ -1287,16 +1289,18 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             StageTimer t(pl, s, 1);
             if ((st = contract<T>(pl, c, cc, true, pl->d_ws.p, (real || c->pt) ? pl->d_fc.p : out, s))) return st;
         }
+        // PT -> packed complex: the output, or (real) the staging in d_tmp that the mixing reads
+        void *packed = real && c->pt ? pl->d_tmp.p : real ? pl->d_fc.p : out;
         if (c->pt) {
             StageTimer t(pl, s, 2);
             const int tgrid = coef_transpose_grid<T2>(nc, nfc);
-            hipLaunchKernelGGL((coef_transpose<0, T2>), dim3(tgrid), dim3(256), 0, s, ca, (int)nc, pl->d_fc.p, out);
+            hipLaunchKernelGGL((coef_transpose<0, T2>), dim3(tgrid), dim3(256), 0, s, cc, (int)nc, pl->d_fc.p, packed);
             HIP_TRY(hipGetLastError());
         }
         if (real) {
             StageTimer t(pl, s, 2);
-            if (s2) hipLaunchKernelGGL((s2_tmix<T, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_fc.p, out, nc);
-            else hipLaunchKernelGGL((so3_tmix<T, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, pl->d_fc.p, out, nc);
+            if (s2) hipLaunchKernelGGL((s2_tmix<T, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, packed, out, nc);
+            else hipLaunchKernelGGL((so3_tmix<T, 0>), dim3(tmix_blocks), dim3(256), 0, s, ca, packed, out, nc);
             HIP_TRY(hipGetLastError());
         }
     } else {
@@ -1309,16 +1313,19 @@ int run_chunk(fftso3_plan_s *pl, Class *c, int i0, int nfc, bool fwd, const void
             }
         }
         if (real && !contracted) {
+            // the mixing's packed complex output: the contraction's input, or (PT) the layout
+            // pass's, staged in d_tmp
+            void *mixed = c->pt ? pl->d_tmp.p : pl->d_fc.p;
             StageTimer t(pl, s, 3);
-            if (s2) hipLaunchKernelGGL((s2_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
-            else hipLaunchKernelGGL((so3_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, pl->d_fc.p, nc);
+            if (s2) hipLaunchKernelGGL((s2_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, mixed, nc);
+            else hipLaunchKernelGGL((so3_tmix<T, 1>), dim3(tmix_blocks), dim3(256), 0, s, ca, in, mixed, nc);
             HIP_TRY(hipGetLastError());
-            csrc = pl->d_fc.p;
+            csrc = mixed;
         }
         if (!contracted && c->pt) {
             StageTimer t(pl, s, 3);
             const int tgrid = coef_transpose_grid<T2>(nc, nfc);
-            hipLaunchKernelGGL((coef_transpose<1, T2>), dim3(tgrid), dim3(256), 0, s, ca, (int)nc, in, pl->d_fc.p);
+            hipLaunchKernelGGL((coef_transpose<1, T2>), dim3(tgrid), dim3(256), 0, s, cc, (int)nc, csrc, pl->d_fc.p);
             HIP_TRY(hipGetLastError());
             csrc = pl->d_fc.p;
         }
